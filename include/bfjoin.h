@@ -1,0 +1,139 @@
+/*
+ * bfjoin.h -- C ABI of the MI355X brute-force Euclidean epsilon self-join.
+ *
+ * This is the drop-in boundary under the reference's host driver.  The
+ * reference (jeevana-swaroop/tensor_bf_main) has one host entry point,
+ *     void GPUJoinMainBruteForce(unsigned searchMode, unsigned device,
+ *                                INPUT_DATA_TYPE *dataset, unsigned *nbQueryPoints,
+ *                                ACCUM_TYPE *epsilon, uint64_t *totalNeighbors);
+ * (gpu_join.h:36-42, body gpu_join.cu:210-322), which allocates device memory,
+ * copies the padded dataset in, runs the prep kernels (convertDataset,
+ * kernel_join.cu:44-54; preComputedSquaredCoordinates, kernel_join.cu:57-88),
+ * launches one join kernel (kernel_join.cu:168-677) and copies back one uint64.
+ * The functions below replace that body; include/gpu_join.h keeps the
+ * reference signature on top of them.
+ *
+ * Conventions: every function returns a bfj_status; on failure the reason is
+ * available from bfj_last_error() (thread-local).  No C++ exception crosses
+ * this boundary.  Plain pointers and sizes only.
+ *
+ * Semantics (kernel_join.cu:184-198, gpu_join.cu:318-321): for every ordered
+ * pair (i, j) of rows, i in [row_begin, row_end), j in [0, n), the pair is a hit
+ * when sqrt(d2(i, j)) <= eps, where d2 is the FP64 direct form
+ * sum_k fma-accumulated (x_i[k] - x_j[k])^2 in ascending k.  counts[] exclude
+ * the self pair; totals[] include it (the reference's totalNeighbors).  Every
+ * dtype returns exactly these counts: low-precision dot products only decide
+ * the pairs whose error band excludes the threshold; the rest are re-evaluated
+ * in FP64 on the GPU.
+ */
+#ifndef BFJOIN_H
+#define BFJOIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BFJ_ABI_VERSION 1
+
+typedef struct bfj_ctx bfj_ctx;
+typedef struct bfj_plan bfj_plan;
+
+typedef enum bfj_status {
+    BFJ_OK = 0,
+    BFJ_ERR_INVALID = 1,     /* bad argument (null pointer, n < 0, d < 1, ld < d, NaN eps, ...) */
+    BFJ_ERR_DEVICE = 2,      /* HIP runtime error or bad device index */
+    BFJ_ERR_NOMEM = 3,       /* device allocation failed */
+    BFJ_ERR_UNSUPPORTED = 4, /* dimension or dtype outside what the kernels are built for */
+    BFJ_ERR_INTERNAL = 5
+} bfj_status;
+
+/* Arithmetic of the dot products.  The band refine is FP64 in every case. */
+typedef enum bfj_dtype {
+    BFJ_DTYPE_FP64 = 0, /* FP64 MFMA (v_mfma_f64_16x16x4), FP64 norms */
+    BFJ_DTYPE_FP16 = 1, /* FP16 MFMA (v_mfma_f32_16x16x32_f16), FP32 accumulation and norms */
+    BFJ_DTYPE_BF16 = 2, /* BF16 MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulation and norms */
+    BFJ_DTYPE_FP8 = 3   /* FP8 e4m3, per-row power-of-two scales (v_mfma_scale_f32_16x16x128_f8f6f4) */
+} bfj_dtype;
+
+#define BFJ_MAX_EPS 8   /* epsilon values per dataset in one pass */
+#define BFJ_MAX_DIM 256 /* largest d the MFMA kernels are built for */
+
+typedef struct bfj_stats {
+    uint64_t pairs;          /* ordered pairs evaluated: (row_end - row_begin) * n, summed over datasets */
+    uint64_t band_pairs;     /* pairs re-evaluated in FP64 (union over eps) */
+    uint64_t band_capacity;  /* refine list capacity used for the pass */
+    int32_t band_retries;    /* passes repeated because the refine list overflowed */
+    int32_t tile_rows;       /* query rows held per wavefront in the chosen tile */
+    int32_t k_padded;        /* dot-product length after zero padding */
+    int32_t dtype;
+    double max_abs_d2_err;   /* max |d2_approx - d2_fp64| over refined pairs, input units */
+    double max_band_halfwidth; /* widest (H - L)/2 over rows and eps, input units */
+    double ms_prep;          /* device time: centre, cast, norms, thresholds */
+    double ms_join;          /* device time: MFMA join with fused epsilon epilogue */
+    double ms_refine;        /* device time: FP64 band refine + count finalisation */
+    double ms_total;         /* device time of the whole pass */
+} bfj_stats;
+
+/* One dataset of a batch.  data/counts are host or device pointers depending on
+ * the call; totals is always host memory. */
+typedef struct bfj_dataset {
+    const double *data; /* n x ld row-major, first d columns used */
+    int64_t n;
+    int32_t d;
+    int64_t ld;
+    int32_t n_eps;
+    const double *eps;  /* [n_eps] host */
+    int64_t *counts;    /* [n_eps][n], self excluded; may be NULL */
+    uint64_t *totals;   /* [n_eps], self included; may be NULL */
+} bfj_dataset;
+
+const char *bfj_last_error(void);
+int bfj_abi_version(void);
+int bfj_device_count(int *count);
+
+/* Context: one per host thread; owns a HIP stream and all device buffers
+ * (the reference leaks them, gpu_join.cu:221-267). */
+int bfj_init(int device, bfj_ctx **ctx);
+int bfj_destroy(bfj_ctx *ctx);
+
+/* Replaces gpu_join.cu:210-322 for one dataset held in host memory.
+ * data: n x ld doubles (host).  counts: [n_eps][row_end-row_begin] (host, may be NULL).
+ * totals: [n_eps] (host, may be NULL).  row_begin/row_end select the query rows
+ * (the shard of one GPU); candidates are always all n rows. */
+int bfj_self_join(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, int64_t ld,
+                  const double *eps, int32_t n_eps, int32_t dtype,
+                  int64_t row_begin, int64_t row_end,
+                  int64_t *counts, uint64_t *totals, bfj_stats *stats);
+
+/* Same with device-resident data and counts (counts: device int64 [n_eps][rows]).
+ * Runs on the context stream and returns after it has synchronised. */
+int bfj_self_join_device(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t d, int64_t ld,
+                         const double *eps, int32_t n_eps, int32_t dtype,
+                         int64_t row_begin, int64_t row_end,
+                         int64_t *d_counts, uint64_t *totals, bfj_stats *stats);
+
+/* Many independent datasets, each with its own eps list, in one persistent launch.
+ * device_resident != 0: sets[i].data / counts are device pointers. */
+int bfj_self_join_batch(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32_t dtype,
+                        int32_t device_resident, bfj_stats *stats);
+
+/* Plans: all buffers allocated, tiles tuned and the whole pass captured in one
+ * hipGraph at creation; bfj_plan_run replays it.  Datasets are device-resident. */
+int bfj_plan_create(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32_t dtype, bfj_plan **plan);
+int bfj_plan_run(bfj_plan *plan, bfj_stats *stats);
+int bfj_plan_destroy(bfj_plan *plan);
+
+/* Tile choice.  bfj_autotune times every built tile shape for (dtype, d) on a
+ * synthetic problem of n points and keeps the fastest in the context; later
+ * joins use it.  bfj_set_tile_rows forces a shape (0 = heuristic). */
+int bfj_autotune(bfj_ctx *ctx, int32_t dtype, int32_t d, int64_t n, int32_t *tile_rows_out);
+int bfj_set_tile_rows(bfj_ctx *ctx, int32_t dtype, int32_t tile_rows);
+int bfj_tile_rows_options(int32_t dtype, int32_t d, int32_t *options, int32_t max_options, int32_t *n_options);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BFJOIN_H */

@@ -1,0 +1,309 @@
+"""MI355X brute-force Euclidean epsilon self-join.
+
+Python plumbing over the C ABI in ``include/bfjoin.h`` (``libbfjoin.so``, built
+from ``csrc/``).  The join itself runs in hand-written gfx950 kernels; this module
+only moves pointers.  There is no CPU fallback: if the library is missing or no
+GPU is present, calls raise.
+
+Reference semantics (jeevana-swaroop/tensor_bf_main): for every ordered pair of
+rows (i, j) the pair is a hit when ``sqrt(d2) <= eps``, with ``d2`` the FP64 direct
+form of kernel_join.cu:184-189; ``GPUJoinMainBruteForce`` (gpu_join.cu:210-322)
+returns the self-inclusive total.  :meth:`Context.self_join` returns that total
+per eps and per-point counts with self excluded.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbfjoin.so")
+CLI_PATH = os.path.join(_HERE, "bfjoin_main")
+
+DTYPES = {"fp64": 0, "fp16": 1, "bf16": 2, "fp8": 3}
+DTYPE_NAMES = {v: k for k, v in DTYPES.items()}
+MAX_EPS = 8
+MAX_DIM = 256
+
+BFJ_OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "DEVICE", 3: "NOMEM", 4: "UNSUPPORTED", 5: "INTERNAL"}
+
+
+class JoinError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bfjoin {STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("pairs", ctypes.c_uint64),
+        ("band_pairs", ctypes.c_uint64),
+        ("band_capacity", ctypes.c_uint64),
+        ("band_retries", ctypes.c_int32),
+        ("tile_rows", ctypes.c_int32),
+        ("k_padded", ctypes.c_int32),
+        ("dtype", ctypes.c_int32),
+        ("max_abs_d2_err", ctypes.c_double),
+        ("max_band_halfwidth", ctypes.c_double),
+        ("ms_prep", ctypes.c_double),
+        ("ms_join", ctypes.c_double),
+        ("ms_refine", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Dataset(ctypes.Structure):
+    _fields_ = [
+        ("data", ctypes.c_void_p),
+        ("n", ctypes.c_int64),
+        ("d", ctypes.c_int32),
+        ("ld", ctypes.c_int64),
+        ("n_eps", ctypes.c_int32),
+        ("eps", ctypes.POINTER(ctypes.c_double)),
+        ("counts", ctypes.c_void_p),
+        ("totals", ctypes.POINTER(ctypes.c_uint64)),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libbfjoin.so.  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() (or make -C csrc)")
+    lib = ctypes.CDLL(path)
+    vp, i32, i64, f64p = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    sp = ctypes.POINTER(Stats)
+    sig = {
+        "bfj_last_error": (ctypes.c_char_p, []),
+        "bfj_abi_version": (ctypes.c_int, []),
+        "bfj_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+        "bfj_init": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+        "bfj_destroy": (ctypes.c_int, [vp]),
+        "bfj_self_join": (ctypes.c_int, [vp, vp, i64, i32, i64, f64p, i32, i32, i64, i64, vp, u64p, sp]),
+        "bfj_self_join_device": (ctypes.c_int, [vp, vp, i64, i32, i64, f64p, i32, i32, i64, i64, vp, u64p, sp]),
+        "bfj_self_join_batch": (ctypes.c_int, [vp, ctypes.POINTER(Dataset), i32, i32, i32, sp]),
+        "bfj_plan_create": (ctypes.c_int, [vp, ctypes.POINTER(Dataset), i32, i32, ctypes.POINTER(vp)]),
+        "bfj_plan_run": (ctypes.c_int, [vp, sp]),
+        "bfj_plan_destroy": (ctypes.c_int, [vp]),
+        "bfj_autotune": (ctypes.c_int, [vp, i32, i32, i64, ctypes.POINTER(i32)]),
+        "bfj_set_tile_rows": (ctypes.c_int, [vp, i32, i32]),
+        "bfj_tile_rows_options": (ctypes.c_int, [i32, i32, ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != BFJ_OK:
+        raise JoinError(rc, _lib.bfj_last_error().decode())
+
+
+def _dtype_id(dtype) -> int:
+    if isinstance(dtype, str):
+        if dtype not in DTYPES:
+            raise ValueError(f"unknown dtype {dtype!r}; choose from {sorted(DTYPES)}")
+        return DTYPES[dtype]
+    return int(dtype)
+
+
+def _eps_array(eps) -> np.ndarray:
+    e = np.ascontiguousarray(np.atleast_1d(np.asarray(eps, dtype=np.float64)))
+    if e.ndim != 1:
+        raise ValueError("eps must be a scalar or a 1-D sequence")
+    return e
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = ctypes.c_int(0)
+    _check(lib.bfj_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def tile_rows_options(dtype, d: int) -> list[int]:
+    lib = load_library()
+    opts = (ctypes.c_int32 * 8)()
+    k = ctypes.c_int32(0)
+    _check(lib.bfj_tile_rows_options(_dtype_id(dtype), d, opts, 8, ctypes.byref(k)))
+    return [opts[i] for i in range(min(k.value, 8))]
+
+
+@dataclass
+class JoinResult:
+    counts: np.ndarray | None   # (n_eps, rows) int64, self excluded
+    totals: np.ndarray          # (n_eps,) uint64, self included (reference totalNeighbors)
+    stats: dict
+
+
+class Context:
+    """One HIP stream and the device buffers of the join (one per host thread)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self.lib.bfj_init(device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.bfj_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- single dataset -------------------------------------------------------------
+    def self_join(self, data, eps, dtype="fp16", row_begin: int = 0, row_end: int | None = None,
+                  counts: bool = True) -> JoinResult:
+        """Join a host (numpy) float64 array of shape (n, d), or a strided view with unit column stride."""
+        x = np.asarray(data)
+        if x.dtype != np.float64 or x.ndim != 2:
+            raise ValueError("data must be a 2-D float64 array")
+        if x.strides[1] != 8 or x.strides[0] % 8:
+            x = np.ascontiguousarray(x)
+        n, d = x.shape
+        ld = x.strides[0] // 8 if n > 1 else d
+        e = _eps_array(eps)
+        if row_end is None:
+            row_end = n
+        rows = max(row_end - row_begin, 0)
+        out = np.zeros((e.size, rows), dtype=np.int64) if counts else None
+        tot = np.zeros(e.size, dtype=np.uint64)
+        st = Stats()
+        _check(self.lib.bfj_self_join(
+            self._h, x.ctypes.data if n else None, n, d, ld, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            e.size, _dtype_id(dtype), row_begin, row_end, out.ctypes.data if counts and rows else None,
+            tot.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(st)))
+        return JoinResult(out, tot, st.as_dict())
+
+    def self_join_device(self, x, eps, dtype="fp16", row_begin: int = 0, row_end: int | None = None,
+                         counts=None) -> JoinResult:
+        """Join a device-resident torch float64 tensor (n, d) with unit column stride.
+
+        ``counts``: optional int64 device tensor of shape (n_eps, rows) that receives
+        the per-point counts."""
+        import torch
+
+        if x.dtype != torch.float64 or x.dim() != 2 or not x.is_cuda or x.stride(1) != 1:
+            raise ValueError("x must be a 2-D float64 CUDA tensor with unit column stride")
+        n, d = x.shape
+        ld = x.stride(0) if n > 1 else d
+        e = _eps_array(eps)
+        if row_end is None:
+            row_end = n
+        if counts is not None:
+            rows = row_end - row_begin
+            if counts.dtype != torch.int64 or tuple(counts.shape) != (e.size, rows) or not counts.is_contiguous():
+                raise ValueError("counts must be a contiguous int64 device tensor of shape (n_eps, rows)")
+        tot = np.zeros(e.size, dtype=np.uint64)
+        st = Stats()
+        torch.cuda.synchronize(x.device)  # producer work on torch's streams must be complete
+        _check(self.lib.bfj_self_join_device(
+            self._h, x.data_ptr() if n else None, n, d, ld, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            e.size, _dtype_id(dtype), row_begin, row_end, counts.data_ptr() if counts is not None else None,
+            tot.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(st)))
+        return JoinResult(None, tot, st.as_dict())
+
+    # -- batches ---------------------------------------------------------------------
+    def _datasets(self, sets, device_resident: bool):
+        arr = (Dataset * len(sets))()
+        keep = []
+        for k, s in enumerate(sets):
+            data, eps = s["data"], _eps_array(s["eps"])
+            tot = np.zeros(eps.size, dtype=np.uint64)
+            if device_resident:
+                n, d = data.shape
+                ptr, ld = data.data_ptr(), data.stride(0) if n > 1 else d
+                cnt = s.get("counts")
+                cptr = cnt.data_ptr() if cnt is not None else None
+            else:
+                data = np.ascontiguousarray(data, dtype=np.float64)
+                n, d = data.shape
+                ptr, ld = data.ctypes.data, d
+                cnt = np.zeros((eps.size, n), dtype=np.int64)
+                cptr = cnt.ctypes.data
+            arr[k] = Dataset(ptr, n, d, ld, eps.size, eps.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), cptr,
+                             tot.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+            keep.append((data, eps, tot, cnt))
+        return arr, keep
+
+    def self_join_batch(self, sets, dtype="fp16"):
+        """sets: list of {"data": (n, d) float64 ndarray, "eps": [...]}.
+        Returns (list of (counts, totals), stats)."""
+        arr, keep = self._datasets(sets, device_resident=False)
+        st = Stats()
+        _check(self.lib.bfj_self_join_batch(self._h, arr, len(sets), _dtype_id(dtype), 0, ctypes.byref(st)))
+        return [(c, t) for (_, _, t, c) in keep], st.as_dict()
+
+    def plan(self, sets, dtype="fp16") -> "Plan":
+        """sets: list of {"data": CUDA float64 tensor, "eps": [...], "counts": optional int64 tensor}."""
+        return Plan(self, sets, dtype)
+
+    # -- tiles -----------------------------------------------------------------------------
+    def autotune(self, dtype, d: int, n: int = 32768) -> int:
+        r = ctypes.c_int32(0)
+        _check(self.lib.bfj_autotune(self._h, _dtype_id(dtype), d, n, ctypes.byref(r)))
+        return r.value
+
+    def set_tile_rows(self, dtype, rows: int):
+        _check(self.lib.bfj_set_tile_rows(self._h, _dtype_id(dtype), rows))
+
+
+class Plan:
+    """A batch captured once into a hipGraph; :meth:`run` replays it."""
+
+    def __init__(self, ctx: Context, sets, dtype="fp16"):
+        import torch
+
+        self.ctx = ctx
+        torch.cuda.synchronize()
+        self._arr, self._keep = ctx._datasets(sets, device_resident=True)
+        h = ctypes.c_void_p()
+        _check(ctx.lib.bfj_plan_create(ctx._h, self._arr, len(sets), _dtype_id(dtype), ctypes.byref(h)))
+        self._h = h
+
+    def run(self) -> dict:
+        st = Stats()
+        _check(self.ctx.lib.bfj_plan_run(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def totals(self):
+        return [t.copy() for (_, _, t, _) in self._keep]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx.lib.bfj_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

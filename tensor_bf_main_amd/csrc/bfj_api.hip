@@ -1,0 +1,906 @@
+// bfj_api.hip -- C ABI and host orchestration of one join pass.
+//
+// Replaces the body of GPUJoinMainBruteForce (gpu_join.cu:210-322):
+//   reference                               here
+//   cudaMalloc + H2D of the padded rows     device buffers owned by a context, grown
+//     (:221-222), never freed                 on demand, freed by bfj_destroy
+//   convertDataset / preComputedSquared-    prep kernels (prep.hip): centre, scale,
+//     Coordinates (:224-255)                  cast into MFMA fragment order, norms,
+//                                             per-row error radii and thresholds
+//   one join kernel per searchMode          MFMA join with fused epsilon epilogue
+//     (:271-316)                              (join_kernel.h), then the FP64 band
+//                                             refine and count finalisation (refine.hip)
+//   cudaDeviceSynchronize + D2H of one      per-row int64 counts + totals per eps;
+//     uint64 (:318-321)                       status codes instead of exit(1)
+#include "bfj_internal.h"
+#include "../../include/bfjoin.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HC(expr)                                                                               \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? BFJ_ERR_NOMEM : BFJ_ERR_DEVICE, "%s: %s", #expr, \
+                        hipGetErrorString(e_));                                                \
+    } while (0)
+
+#define RC(expr)                   \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != BFJ_OK) return r_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// dispatch to the per-dtype translation units
+int bfj_join_launch_f16(const JoinLaunch &, const JoinArgs &, hipStream_t);
+int bfj_join_launch_bf16(const JoinLaunch &, const JoinArgs &, hipStream_t);
+int bfj_join_launch_f8(const JoinLaunch &, const JoinArgs &, hipStream_t);
+int bfj_join_launch_f64(const JoinLaunch &, const JoinArgs &, hipStream_t);
+bool bfj_join_has_f16(int, int);
+bool bfj_join_has_bf16(int, int);
+bool bfj_join_has_f8(int, int);
+bool bfj_join_has_f64(int, int);
+int bfj_join_occ_f16(int, int, int);
+int bfj_join_occ_bf16(int, int, int);
+int bfj_join_occ_f8(int, int, int);
+int bfj_join_occ_f64(int, int, int);
+
+int bfj_join_launch(const JoinLaunch &L, const JoinArgs &a, hipStream_t s)
+{
+    switch (L.dtype) {
+    case BFJ_FP16: return bfj_join_launch_f16(L, a, s);
+    case BFJ_BF16: return bfj_join_launch_bf16(L, a, s);
+    case BFJ_FP8: return bfj_join_launch_f8(L, a, s);
+    case BFJ_FP64: return bfj_join_launch_f64(L, a, s);
+    }
+    return 1;
+}
+
+bool bfj_join_has(int dtype, int kc, int rows)
+{
+    switch (dtype) {
+    case BFJ_FP16: return bfj_join_has_f16(kc, rows);
+    case BFJ_BF16: return bfj_join_has_bf16(kc, rows);
+    case BFJ_FP8: return bfj_join_has_f8(kc, rows);
+    case BFJ_FP64: return bfj_join_has_f64(kc, rows);
+    }
+    return false;
+}
+
+int bfj_join_max_blocks_per_cu(int dtype, int kc, int rows, int lds_bytes)
+{
+    switch (dtype) {
+    case BFJ_FP16: return bfj_join_occ_f16(kc, rows, lds_bytes);
+    case BFJ_BF16: return bfj_join_occ_bf16(kc, rows, lds_bytes);
+    case BFJ_FP8: return bfj_join_occ_f8(kc, rows, lds_bytes);
+    case BFJ_FP64: return bfj_join_occ_f64(kc, rows, lds_bytes);
+    }
+    return 0;
+}
+
+static const int kRowOptions[] = {16, 32, 64, 128};
+
+// Smallest built K-chunk count >= need for this dtype (0 if none).
+static int pick_kc(int dtype, int d)
+{
+    const int need = (d + bfj_kchunk(dtype) - 1) / bfj_kchunk(dtype);
+    for (int kc = need; kc <= 32; ++kc)
+        for (int r : kRowOptions)
+            if (bfj_join_has(dtype, kc, r)) return kc;
+    return 0;
+}
+
+static int lds_for(int dtype, int rows)
+{
+    const int per_wave = BFJ_MAXEPS * rows * 4 + rows * 16 * bfj_acc_bytes(dtype);
+    return per_wave * (BFJ_BLOCK / BFJ_WAVE);
+}
+
+// Default tile: the widest strip whose kernel still runs 2 waves per SIMD,
+// else the widest that was built.
+static int heuristic_rows(int dtype, int kc)
+{
+    int best = 0;
+    for (int r : kRowOptions)
+        if (bfj_join_has(dtype, kc, r)) {
+            const int occ = bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, r));
+            if (occ >= 2 || best == 0) best = r;
+        }
+    return best;
+}
+
+// Largest double T with sqrt(T) <= eps: the predicate of kernel_join.cu:194
+// (sqrt(d2) <= eps) becomes d2 <= T.  eps < 0 matches nothing (-1), eps = inf
+// matches everything.
+static double eps_threshold(double eps)
+{
+    if (eps < 0.0) return -1.0;
+    if (std::isinf(eps)) return INFINITY;
+    double t = eps * eps;
+    if (std::isinf(t)) t = 1.7976931348623157e308;
+    while (t > 0.0 && std::sqrt(t) > eps) t = std::nextafter(t, -INFINITY);
+    for (;;) {
+        const double u = std::nextafter(t, INFINITY);
+        if (std::isinf(u) || std::sqrt(u) > eps) break;
+        t = u;
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// context
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct bfj_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[5] = {};
+    DevBuf pack, nq, gq, ov, hmax, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
+        outbuf;
+    uint32_t band_capacity = 0;
+    int tile_rows[4][33] = {}; // [dtype][kc] forced/tuned R (0 = heuristic)
+};
+
+static int ensure(DevBuf &b, size_t bytes)
+{
+    if (bytes <= b.cap) return BFJ_OK;
+    if (b.p) HC(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 8 + 256;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        return fail(BFJ_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+    }
+    b.cap = want;
+    return BFJ_OK;
+}
+
+static void release(DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// ---------------------------------------------------------------------------
+// one pass over a set of datasets
+
+struct HostSet {
+    const double *x; // device
+    int64_t n, ld, row_begin, row_end;
+    int32_t d, n_eps;
+    double eps[BFJ_MAXEPS];
+    int64_t *out; // device or NULL
+};
+
+struct Group {
+    int kc, rows, lds;
+    int32_t item0, n_items;
+};
+
+struct PassPlan {
+    int dtype = 0;
+    std::vector<SetDesc> sets;
+    std::vector<WorkItem> items;
+    std::vector<Group> groups;
+    std::vector<int32_t> blk_set, set_blk0, set_nblk;
+    std::vector<int64_t> blk_row0;
+    int64_t rows_total = 0, pack_bytes = 0, ov_total = 0, cnt_total = 0;
+    uint64_t pairs = 0;
+    int max_rows = 0, max_kc = 0;
+};
+
+static int validate(const HostSet &h, int dtype)
+{
+    if (!h.x && h.n > 0) return fail(BFJ_ERR_INVALID, "data is NULL");
+    if (h.n < 0) return fail(BFJ_ERR_INVALID, "n < 0");
+    if (h.n > (int64_t)0x7fffffff) return fail(BFJ_ERR_UNSUPPORTED, "n > 2^31-1");
+    if (h.d < 1) return fail(BFJ_ERR_INVALID, "d < 1");
+    if (h.d > BFJ_MAX_DIM) return fail(BFJ_ERR_UNSUPPORTED, "d = %d > %d", h.d, BFJ_MAX_DIM);
+    if (h.ld < h.d) return fail(BFJ_ERR_INVALID, "ld < d");
+    if (h.n_eps < 1 || h.n_eps > BFJ_MAXEPS) return fail(BFJ_ERR_INVALID, "n_eps must be in [1, %d]", BFJ_MAXEPS);
+    for (int e = 0; e < h.n_eps; ++e)
+        if (std::isnan(h.eps[e])) return fail(BFJ_ERR_INVALID, "eps[%d] is NaN", e);
+    if (h.row_begin < 0 || h.row_end > h.n || h.row_begin > h.row_end)
+        return fail(BFJ_ERR_INVALID, "row range [%lld, %lld) outside [0, %lld]", (long long)h.row_begin,
+                    (long long)h.row_end, (long long)h.n);
+    if (dtype < BFJ_FP64 || dtype > BFJ_FP8) return fail(BFJ_ERR_INVALID, "unknown dtype %d", dtype);
+    if (!pick_kc(dtype, h.d)) return fail(BFJ_ERR_UNSUPPORTED, "no kernel built for dtype %d, d = %d", dtype, h.d);
+    return BFJ_OK;
+}
+
+static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, PassPlan &p, int force_rows)
+{
+    p = PassPlan();
+    p.dtype = dtype;
+    const int FB = bfj_frag_bytes(dtype);
+    for (size_t si = 0; si < hs.size(); ++si) {
+        const HostSet &h = hs[si];
+        RC(validate(h, dtype));
+        SetDesc s;
+        memset(&s, 0, sizeof s);
+        s.x = h.x;
+        s.n = h.n;
+        s.npad = (h.n + 15) / 16 * 16;
+        s.ld = h.ld;
+        s.row_begin = h.row_begin;
+        s.row_end = h.row_end;
+        s.d = h.d;
+        s.kc = pick_kc(dtype, h.d);
+        s.n_eps = h.n_eps;
+        s.set_id = (int32_t)si;
+        s.out = h.out;
+        for (int e = 0; e < h.n_eps; ++e) s.T[e] = eps_threshold(h.eps[e]);
+        s.row_off = p.rows_total;
+        s.pack_off = p.pack_bytes;
+        s.ov_off = p.ov_total;
+        s.cnt_off = p.cnt_total;
+        p.rows_total += s.npad;
+        p.pack_bytes += (s.npad / 16) * (int64_t)s.kc * BFJ_WAVE * FB;
+        p.ov_total += s.npad * s.n_eps;
+        p.cnt_total += s.npad * s.n_eps;
+        p.pairs += (uint64_t)(h.row_end - h.row_begin) * (uint64_t)h.n;
+        p.max_kc = std::max(p.max_kc, s.kc);
+        // Row blocks for the row-parallel prep / finalize kernels.
+        p.set_blk0.push_back((int32_t)p.blk_set.size());
+        int32_t nb = 0;
+        for (int64_t r0 = 0; r0 < s.npad; r0 += BFJ_PREP_ROWS, ++nb) {
+            p.blk_set.push_back((int32_t)si);
+            p.blk_row0.push_back(r0);
+        }
+        p.set_nblk.push_back(nb);
+        p.sets.push_back(s);
+    }
+    if (p.sets.size() > 65535) return fail(BFJ_ERR_UNSUPPORTED, "more than 65535 datasets in one pass");
+    // Work items grouped by K-chunk count, longest strips first (greedy LPT through the queue).
+    std::vector<int> kcs;
+    for (auto &s : p.sets)
+        if (std::find(kcs.begin(), kcs.end(), s.kc) == kcs.end()) kcs.push_back(s.kc);
+    std::sort(kcs.begin(), kcs.end());
+    for (int kc : kcs) {
+        Group g;
+        g.kc = kc;
+        int rows = force_rows ? force_rows : ctx->tile_rows[dtype][kc];
+        if (!rows || !bfj_join_has(dtype, kc, rows)) rows = heuristic_rows(dtype, kc);
+        if (!bfj_join_has(dtype, kc, rows)) return fail(BFJ_ERR_UNSUPPORTED, "no tile for dtype %d kc %d", dtype, kc);
+        g.rows = rows;
+        g.lds = lds_for(dtype, rows);
+        g.item0 = (int32_t)p.items.size();
+        std::vector<std::pair<int64_t, WorkItem>> its;
+        for (size_t si = 0; si < p.sets.size(); ++si) {
+            const SetDesc &s = p.sets[si];
+            if (s.kc != kc || s.row_end <= s.row_begin) continue;
+            const int64_t rb_begin = s.row_begin / 16, rb_end = (s.row_end + 15) / 16;
+            for (int64_t rb = rb_begin; rb < rb_end; rb += rows / 16) {
+                WorkItem w;
+                w.set = (int32_t)si;
+                w.rb0 = (int32_t)rb;
+                its.push_back({s.npad, w});
+            }
+        }
+        std::stable_sort(its.begin(), its.end(), [](const std::pair<int64_t, WorkItem> &a,
+                                                    const std::pair<int64_t, WorkItem> &b) { return a.first > b.first; });
+        for (auto &x : its) p.items.push_back(x.second);
+        g.n_items = (int32_t)(p.items.size() - g.item0);
+        p.max_rows = std::max(p.max_rows, rows);
+        if (g.n_items) p.groups.push_back(g);
+    }
+    return BFJ_OK;
+}
+
+// Buffers sized for the plan; descriptors uploaded.
+static int stage_plan(bfj_ctx *ctx, PassPlan &p)
+{
+    const int dt = p.dtype;
+    const size_t vb = bfj_acc_bytes(dt);
+    const size_t nrows = (size_t)std::max<int64_t>(p.rows_total, 1);
+    RC(ensure(ctx->pack, std::max<int64_t>(p.pack_bytes, 64)));
+    RC(ensure(ctx->nq, nrows * vb));
+    RC(ensure(ctx->gq, nrows * vb));
+    RC(ensure(ctx->ov, std::max<int64_t>(p.ov_total, 1) * 2 * vb));
+    RC(ensure(ctx->hmax, nrows * 8));
+    RC(ensure(ctx->qerr, nrows * 8));
+    RC(ensure(ctx->nrm, nrows * 8));
+    RC(ensure(ctx->scales, nrows));
+    RC(ensure(ctx->cnt, std::max<int64_t>(p.cnt_total, 1) * 4));
+    if (ctx->band_capacity == 0) {
+        const uint64_t want = std::min<uint64_t>(std::max<uint64_t>((uint64_t)p.rows_total * 64, 1u << 20), 1u << 28);
+        ctx->band_capacity = (uint32_t)want;
+    }
+    RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+    RC(ensure(ctx->counters, 64 * 4));
+    RC(ensure(ctx->colsum, std::max<size_t>(p.blk_set.size(), 1) * BFJ_MAXDIM * 8));
+    RC(ensure(ctx->mu, p.sets.size() * BFJ_MAXDIM * 8));
+    RC(ensure(ctx->sets, p.sets.size() * sizeof(SetDesc)));
+    RC(ensure(ctx->scal, p.sets.size() * sizeof(SetScalars)));
+    RC(ensure(ctx->items, std::max<size_t>(p.items.size(), 1) * sizeof(WorkItem)));
+    const size_t nb = p.blk_set.size(), ns = p.sets.size();
+    const size_t tab_bytes = nb * 4 + nb * 8 + ns * 4 * 2 + 64;
+    RC(ensure(ctx->tabs, tab_bytes));
+    HC(hipMemcpyAsync(ctx->sets.p, p.sets.data(), ns * sizeof(SetDesc), hipMemcpyHostToDevice, ctx->stream));
+    if (!p.items.empty())
+        HC(hipMemcpyAsync(ctx->items.p, p.items.data(), p.items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                          ctx->stream));
+    unsigned char *t = (unsigned char *)ctx->tabs.p;
+    HC(hipMemcpyAsync(t, p.blk_row0.data(), nb * 8, hipMemcpyHostToDevice, ctx->stream));
+    HC(hipMemcpyAsync(t + nb * 8, p.blk_set.data(), nb * 4, hipMemcpyHostToDevice, ctx->stream));
+    HC(hipMemcpyAsync(t + nb * 12, p.set_blk0.data(), ns * 4, hipMemcpyHostToDevice, ctx->stream));
+    HC(hipMemcpyAsync(t + nb * 12 + ns * 4, p.set_nblk.data(), ns * 4, hipMemcpyHostToDevice, ctx->stream));
+    return BFJ_OK;
+}
+
+static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
+{
+    PassArgs a;
+    memset(&a, 0, sizeof a);
+    const size_t nb = p.blk_set.size(), ns = p.sets.size();
+    unsigned char *t = (unsigned char *)ctx->tabs.p;
+    a.sets = (const SetDesc *)ctx->sets.p;
+    a.scal = (SetScalars *)ctx->scal.p;
+    a.n_sets = (int32_t)ns;
+    a.dtype = p.dtype;
+    a.blk_row0 = (const int64_t *)t;
+    a.blk_set = (const int32_t *)(t + nb * 8);
+    a.set_blk0 = (const int32_t *)(t + nb * 12);
+    a.set_nblk = (const int32_t *)(t + nb * 12 + ns * 4);
+    a.n_row_blocks = (int32_t)nb;
+    a.colsum = (double *)ctx->colsum.p;
+    a.mu = (double *)ctx->mu.p;
+    a.pack = (unsigned char *)ctx->pack.p;
+    a.nq = ctx->nq.p;
+    a.gq = ctx->gq.p;
+    a.ov = ctx->ov.p;
+    a.hmax = (double *)ctx->hmax.p;
+    a.qerr = (double *)ctx->qerr.p;
+    a.nrm = (double *)ctx->nrm.p;
+    a.scales = (uint8_t *)ctx->scales.p;
+    a.cnt = (uint32_t *)ctx->cnt.p;
+    a.cnt_len = p.cnt_total;
+    a.band = (BandEntry *)ctx->band.p;
+    a.band_count = (uint32_t *)ctx->counters.p;
+    a.band_capacity = ctx->band_capacity;
+    return a;
+}
+
+// Enqueue the whole pass on the context stream (no host synchronisation, no
+// memset nodes: safe to capture in a graph).
+static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
+{
+    const PassArgs a = pass_args(ctx, p);
+    hipStream_t st = ctx->stream;
+    if (timed) HC(hipEventRecord(ctx->ev[0], st));
+    bfj_launch_zero(ctx->cnt.p, (size_t)std::max<int64_t>(p.cnt_total, 1) * 4, st);
+    bfj_launch_zero(ctx->scal.p, p.sets.size() * sizeof(SetScalars), st);
+    bfj_launch_zero(ctx->counters.p, 64 * 4, st);
+    if (p.blk_set.size()) bfj_launch_prep(a, st);
+    HC(hipGetLastError());
+    if (timed) HC(hipEventRecord(ctx->ev[1], st));
+    uint32_t *item_ctr = (uint32_t *)ctx->counters.p + 16;
+    for (size_t gi = 0; gi < p.groups.size(); ++gi) {
+        const Group &g = p.groups[gi];
+        JoinArgs j;
+        memset(&j, 0, sizeof j);
+        j.sets = a.sets;
+        j.items = (const WorkItem *)ctx->items.p + g.item0;
+        j.n_items = g.n_items;
+        j.item_counter = item_ctr + gi % 32;
+        j.pack = a.pack;
+        j.nq = a.nq;
+        j.gq = a.gq;
+        j.ov = a.ov;
+        j.scales = a.scales;
+        j.cnt = a.cnt;
+        j.band = a.band;
+        j.band_count = a.band_count;
+        j.band_capacity = a.band_capacity;
+        JoinLaunch L;
+        L.dtype = p.dtype;
+        L.kc = g.kc;
+        L.rows = g.rows;
+        L.lds_bytes = g.lds;
+        const int occ = bfj_join_max_blocks_per_cu(p.dtype, g.kc, g.rows, g.lds);
+        const int waves_needed = g.n_items;
+        const int persistent = std::max(1, occ) * ctx->n_cu;
+        L.grid_blocks = std::min(persistent, (waves_needed + 3) / 4);
+        if (L.grid_blocks < 1) L.grid_blocks = 1;
+        const int rc = bfj_join_launch(L, j, st);
+        if (rc == 1) return fail(BFJ_ERR_INTERNAL, "join kernel dtype %d kc %d rows %d not built", p.dtype, g.kc, g.rows);
+        if (rc) return fail(BFJ_ERR_DEVICE, "join launch failed: %s", hipGetErrorString(hipGetLastError()));
+    }
+    if (timed) HC(hipEventRecord(ctx->ev[2], st));
+    bfj_launch_refine(a, st);
+    bfj_launch_finalize(a, st);
+    HC(hipGetLastError());
+    if (timed) HC(hipEventRecord(ctx->ev[3], st));
+    return BFJ_OK;
+}
+
+static double bits_to_double(unsigned long long b)
+{
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+
+// Run a staged pass to completion, growing the refine list and repeating the
+// pass when it overflowed.  Totals per set land in totals_out[set][eps].
+static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>> *totals_out, bfj_stats *stats,
+                    bool timed = true)
+{
+    int retries = 0;
+    for (;;) {
+        RC(enqueue_pass(ctx, p, timed));
+        uint32_t band_count = 0;
+        HC(hipMemcpyAsync(&band_count, ctx->counters.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipStreamSynchronize(ctx->stream));
+        if (band_count <= ctx->band_capacity) {
+            std::vector<SetScalars> sc(p.sets.size());
+            HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
+            if (totals_out) {
+                totals_out->assign(p.sets.size(), {});
+                for (size_t si = 0; si < p.sets.size(); ++si)
+                    (*totals_out)[si].assign(sc[si].total, sc[si].total + p.sets[si].n_eps);
+            }
+            if (stats) {
+                memset(stats, 0, sizeof *stats);
+                stats->pairs = p.pairs;
+                stats->band_pairs = band_count;
+                stats->band_capacity = ctx->band_capacity;
+                stats->band_retries = retries;
+                stats->tile_rows = p.max_rows;
+                stats->k_padded = p.max_kc * bfj_kchunk(p.dtype);
+                stats->dtype = p.dtype;
+                for (auto &s : sc) {
+                    stats->max_abs_d2_err = std::max(stats->max_abs_d2_err, bits_to_double(s.err_bits));
+                    stats->max_band_halfwidth = std::max(stats->max_band_halfwidth, bits_to_double(s.halfband_bits));
+                }
+                if (timed) {
+                    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+                    HC(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+                    HC(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+                    HC(hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
+                    HC(hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
+                    stats->ms_prep = t01;
+                    stats->ms_join = t12;
+                    stats->ms_refine = t23;
+                    stats->ms_total = t03;
+                }
+            }
+            return BFJ_OK;
+        }
+        // Overflow: every pair past the capacity was dropped; grow and repeat the pass.
+        if (++retries > 3) return fail(BFJ_ERR_INTERNAL, "refine list keeps overflowing (%u entries)", band_count);
+        const uint64_t want = std::min<uint64_t>((uint64_t)band_count + band_count / 4 + 1024, 0xffffffffull);
+        ctx->band_capacity = (uint32_t)want;
+        RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+    }
+}
+
+// Uniform [0, 1) rows for the tile autotuner (splitmix64).
+__global__ void k_synth(double *x, int64_t total, uint64_t seed)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)total; i += stride) {
+        uint64_t z = seed + 0x9e3779b97f4a7c15ull * (i + 1);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        x[i] = (double)(z >> 11) * 0x1p-53;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+const char *bfj_last_error(void) { return g_err.c_str(); }
+
+int bfj_abi_version(void) { return BFJ_ABI_VERSION; }
+
+int bfj_device_count(int *count)
+{
+    if (!count) return fail(BFJ_ERR_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return BFJ_OK;
+}
+
+int bfj_init(int device, bfj_ctx **out)
+{
+    if (!out) return fail(BFJ_ERR_INVALID, "ctx out-pointer is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return fail(BFJ_ERR_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= n) return fail(BFJ_ERR_DEVICE, "device %d out of range [0, %d)", device, n);
+    HC(hipSetDevice(device));
+    bfj_ctx *c = new bfj_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        for (auto &ev : c->ev)
+            if (e == hipSuccess) e = hipEventCreate(&ev);
+    }
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(BFJ_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return BFJ_OK;
+}
+
+int bfj_destroy(bfj_ctx *c)
+{
+    if (!c) return BFJ_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->ov, &c->hmax, &c->qerr, &c->nrm, &c->scales, &c->cnt,
+                      &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
+                      &c->xcopy, &c->outbuf};
+    for (DevBuf *b : bufs) release(*b);
+    for (auto &ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BFJ_OK;
+}
+
+static int join_one(bfj_ctx *ctx, const double *d_x, int64_t n, int32_t d, int64_t ld, const double *eps,
+                    int32_t n_eps, int32_t dtype, int64_t row_begin, int64_t row_end, int64_t *d_counts,
+                    uint64_t *totals, bfj_stats *stats, int force_rows)
+{
+    HostSet h;
+    memset(&h, 0, sizeof h);
+    h.x = d_x;
+    h.n = n;
+    h.d = d;
+    h.ld = ld;
+    h.row_begin = row_begin;
+    h.row_end = row_end;
+    h.n_eps = n_eps;
+    h.out = d_counts;
+    for (int e = 0; e < n_eps && e < BFJ_MAXEPS; ++e) h.eps[e] = eps[e];
+    PassPlan p;
+    RC(build_plan(ctx, {h}, dtype, p, force_rows));
+    RC(stage_plan(ctx, p));
+    std::vector<std::vector<uint64_t>> tot;
+    RC(run_pass(ctx, p, &tot, stats));
+    if (totals)
+        for (int e = 0; e < n_eps; ++e) totals[e] = tot[0][e];
+    return BFJ_OK;
+}
+
+static int common_checks(bfj_ctx *ctx, const double *data, int64_t n, const double *eps, int32_t n_eps)
+{
+    if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
+    if (!eps) return fail(BFJ_ERR_INVALID, "eps is NULL");
+    if (n_eps < 1 || n_eps > BFJ_MAXEPS) return fail(BFJ_ERR_INVALID, "n_eps must be in [1, %d]", BFJ_MAXEPS);
+    if (n < 0) return fail(BFJ_ERR_INVALID, "n < 0");
+    if (n > 0 && !data) return fail(BFJ_ERR_INVALID, "data is NULL");
+    HC(hipSetDevice(ctx->device));
+    return BFJ_OK;
+}
+
+int bfj_self_join_device(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t d, int64_t ld, const double *eps,
+                         int32_t n_eps, int32_t dtype, int64_t row_begin, int64_t row_end, int64_t *d_counts,
+                         uint64_t *totals, bfj_stats *stats)
+{
+    RC(common_checks(ctx, d_data, n, eps, n_eps));
+    return join_one(ctx, d_data, n, d, ld, eps, n_eps, dtype, row_begin, row_end, d_counts, totals, stats, 0);
+}
+
+int bfj_self_join(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, int64_t ld, const double *eps,
+                  int32_t n_eps, int32_t dtype, int64_t row_begin, int64_t row_end, int64_t *counts,
+                  uint64_t *totals, bfj_stats *stats)
+{
+    RC(common_checks(ctx, data, n, eps, n_eps));
+    if (d < 1 || ld < d) return fail(BFJ_ERR_INVALID, "need 1 <= d <= ld");
+    if (row_begin < 0 || row_end > n || row_begin > row_end) return fail(BFJ_ERR_INVALID, "bad row range");
+    const size_t xbytes = (size_t)std::max<int64_t>(n, 1) * (size_t)ld * 8;
+    RC(ensure(ctx->xcopy, xbytes));
+    if (n > 0) HC(hipMemcpyAsync(ctx->xcopy.p, data, (size_t)n * ld * 8, hipMemcpyHostToDevice, ctx->stream));
+    const size_t obytes = (size_t)std::max<int64_t>(row_end - row_begin, 1) * n_eps * 8;
+    RC(ensure(ctx->outbuf, obytes));
+    RC(join_one(ctx, (const double *)ctx->xcopy.p, n, d, ld, eps, n_eps, dtype, row_begin, row_end,
+                counts ? (int64_t *)ctx->outbuf.p : nullptr, totals, stats, 0));
+    if (counts && row_end > row_begin)
+        HC(hipMemcpy(counts, ctx->outbuf.p, (size_t)(row_end - row_begin) * n_eps * 8, hipMemcpyDeviceToHost));
+    return BFJ_OK;
+}
+
+static int batch_hostsets(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int device_resident,
+                          std::vector<HostSet> &hs, std::vector<size_t> &xoff, std::vector<size_t> &ooff)
+{
+    if (!sets || n_sets < 1) return fail(BFJ_ERR_INVALID, "no datasets");
+    hs.clear();
+    xoff.clear();
+    ooff.clear();
+    size_t xb = 0, ob = 0;
+    for (int i = 0; i < n_sets; ++i) {
+        const bfj_dataset &s = sets[i];
+        if (!s.eps || s.n_eps < 1 || s.n_eps > BFJ_MAXEPS)
+            return fail(BFJ_ERR_INVALID, "dataset %d: eps missing or n_eps outside [1, %d]", i, BFJ_MAXEPS);
+        HostSet h;
+        memset(&h, 0, sizeof h);
+        h.x = s.data;
+        h.n = s.n;
+        h.d = s.d;
+        h.ld = s.ld;
+        h.row_begin = 0;
+        h.row_end = s.n;
+        h.n_eps = s.n_eps;
+        h.out = device_resident ? s.counts : nullptr;
+        for (int e = 0; e < s.n_eps; ++e) h.eps[e] = s.eps[e];
+        hs.push_back(h);
+        xoff.push_back(xb);
+        ooff.push_back(ob);
+        if (s.n > 0 && s.ld > 0) xb += (size_t)s.n * (size_t)s.ld * 8;
+        if (s.n > 0) ob += (size_t)s.n * s.n_eps * 8;
+    }
+    (void)ctx;
+    return BFJ_OK;
+}
+
+int bfj_self_join_batch(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32_t dtype, int32_t device_resident,
+                        bfj_stats *stats)
+{
+    if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
+    HC(hipSetDevice(ctx->device));
+    std::vector<HostSet> hs;
+    std::vector<size_t> xoff, ooff;
+    RC(batch_hostsets(ctx, sets, n_sets, device_resident, hs, xoff, ooff));
+    if (!device_resident) {
+        // Copy every dataset into one device buffer; counts come back through another.
+        size_t xb = 0, ob = 0;
+        for (int i = 0; i < n_sets; ++i) {
+            if (sets[i].n > 0 && sets[i].ld > 0) xb = xoff[i] + (size_t)sets[i].n * sets[i].ld * 8;
+            if (sets[i].n > 0) ob = ooff[i] + (size_t)sets[i].n * sets[i].n_eps * 8;
+        }
+        RC(ensure(ctx->xcopy, std::max<size_t>(xb, 8)));
+        RC(ensure(ctx->outbuf, std::max<size_t>(ob, 8)));
+        for (int i = 0; i < n_sets; ++i) {
+            if (sets[i].n > 0 && !sets[i].data) return fail(BFJ_ERR_INVALID, "dataset %d: data is NULL", i);
+            if (sets[i].n > 0 && sets[i].ld >= sets[i].d && sets[i].d >= 1)
+                HC(hipMemcpyAsync((char *)ctx->xcopy.p + xoff[i], sets[i].data, (size_t)sets[i].n * sets[i].ld * 8,
+                                  hipMemcpyHostToDevice, ctx->stream));
+            hs[i].x = (const double *)((char *)ctx->xcopy.p + xoff[i]);
+            hs[i].out = sets[i].counts ? (int64_t *)((char *)ctx->outbuf.p + ooff[i]) : nullptr;
+        }
+    }
+    PassPlan p;
+    RC(build_plan(ctx, hs, dtype, p, 0));
+    RC(stage_plan(ctx, p));
+    std::vector<std::vector<uint64_t>> tot;
+    RC(run_pass(ctx, p, &tot, stats));
+    for (int i = 0; i < n_sets; ++i) {
+        if (sets[i].totals)
+            for (int e = 0; e < sets[i].n_eps; ++e) sets[i].totals[e] = tot[i][e];
+        if (!device_resident && sets[i].counts && sets[i].n > 0)
+            HC(hipMemcpy(sets[i].counts, (char *)ctx->outbuf.p + ooff[i], (size_t)sets[i].n * sets[i].n_eps * 8,
+                         hipMemcpyDeviceToHost));
+    }
+    return BFJ_OK;
+}
+
+// ---- plans: one hipGraph per batch ---------------------------------------------
+struct bfj_plan {
+    bfj_ctx *ctx = nullptr;
+    PassPlan pass;
+    std::vector<bfj_dataset> sets;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t captured_capacity = 0;
+};
+
+static int capture(bfj_plan *pl)
+{
+    bfj_ctx *ctx = pl->ctx;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    pl->exec = nullptr;
+    pl->graph = nullptr;
+    HC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_pass(ctx, pl->pass, true);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != BFJ_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(BFJ_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    pl->graph = g;
+    HC(hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0));
+    pl->captured_capacity = ctx->band_capacity;
+    return BFJ_OK;
+}
+
+int bfj_plan_create(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32_t dtype, bfj_plan **out)
+{
+    if (!ctx || !out) return fail(BFJ_ERR_INVALID, "ctx or plan out-pointer is NULL");
+    *out = nullptr;
+    HC(hipSetDevice(ctx->device));
+    std::vector<HostSet> hs;
+    std::vector<size_t> xoff, ooff;
+    RC(batch_hostsets(ctx, sets, n_sets, 1, hs, xoff, ooff));
+    bfj_plan *pl = new bfj_plan();
+    pl->ctx = ctx;
+    pl->sets.assign(sets, sets + n_sets);
+    int rc = build_plan(ctx, hs, dtype, pl->pass, 0);
+    if (rc == BFJ_OK) rc = stage_plan(ctx, pl->pass);
+    if (rc == BFJ_OK) {
+        // One eager pass sizes the refine list, then the pass is captured.
+        rc = run_pass(ctx, pl->pass, nullptr, nullptr);
+    }
+    if (rc == BFJ_OK) rc = capture(pl);
+    if (rc != BFJ_OK) {
+        bfj_plan_destroy(pl);
+        return rc;
+    }
+    *out = pl;
+    return BFJ_OK;
+}
+
+int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
+{
+    if (!pl) return fail(BFJ_ERR_INVALID, "plan is NULL");
+    bfj_ctx *ctx = pl->ctx;
+    HC(hipSetDevice(ctx->device));
+    // The context's buffers are shared with eager calls: restage if they moved.
+    RC(stage_plan(ctx, pl->pass));
+    if (pl->captured_capacity != ctx->band_capacity) RC(capture(pl));
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        HC(hipGraphLaunch(pl->exec, ctx->stream));
+        uint32_t band_count = 0;
+        HC(hipMemcpyAsync(&band_count, ctx->counters.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipStreamSynchronize(ctx->stream));
+        if (band_count <= ctx->band_capacity) {
+            std::vector<SetScalars> sc(pl->pass.sets.size());
+            HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < pl->sets.size(); ++i)
+                if (pl->sets[i].totals)
+                    for (int e = 0; e < pl->sets[i].n_eps; ++e) pl->sets[i].totals[e] = sc[i].total[e];
+            if (stats) {
+                memset(stats, 0, sizeof *stats);
+                stats->pairs = pl->pass.pairs;
+                stats->band_pairs = band_count;
+                stats->band_capacity = ctx->band_capacity;
+                stats->band_retries = attempt;
+                stats->tile_rows = pl->pass.max_rows;
+                stats->k_padded = pl->pass.max_kc * bfj_kchunk(pl->pass.dtype);
+                stats->dtype = pl->pass.dtype;
+                for (auto &s : sc) {
+                    stats->max_abs_d2_err = std::max(stats->max_abs_d2_err, bits_to_double(s.err_bits));
+                    stats->max_band_halfwidth = std::max(stats->max_band_halfwidth, bits_to_double(s.halfband_bits));
+                }
+                float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+                HC(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+                HC(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+                HC(hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
+                HC(hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
+                stats->ms_prep = t01;
+                stats->ms_join = t12;
+                stats->ms_refine = t23;
+                stats->ms_total = t03;
+            }
+            return BFJ_OK;
+        }
+        ctx->band_capacity = (uint32_t)std::min<uint64_t>((uint64_t)band_count + band_count / 4 + 1024, 0xffffffffull);
+        RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+        RC(capture(pl));
+    }
+    return fail(BFJ_ERR_INTERNAL, "refine list keeps overflowing");
+}
+
+int bfj_plan_destroy(bfj_plan *pl)
+{
+    if (!pl) return BFJ_OK;
+    if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void)hipGraphDestroy(pl->graph);
+    delete pl;
+    return BFJ_OK;
+}
+
+// ---- tiles ------------------------------------------------------------------------
+int bfj_tile_rows_options(int32_t dtype, int32_t d, int32_t *options, int32_t max_options, int32_t *n_options)
+{
+    if (!n_options) return fail(BFJ_ERR_INVALID, "n_options is NULL");
+    *n_options = 0;
+    if (dtype < BFJ_FP64 || dtype > BFJ_FP8) return fail(BFJ_ERR_INVALID, "unknown dtype %d", dtype);
+    if (d < 1 || d > BFJ_MAX_DIM) return fail(BFJ_ERR_UNSUPPORTED, "d = %d outside [1, %d]", d, BFJ_MAX_DIM);
+    const int kc = pick_kc(dtype, d);
+    int k = 0;
+    for (int r : kRowOptions)
+        if (kc && bfj_join_has(dtype, kc, r)) {
+            if (options && k < max_options) options[k] = r;
+            ++k;
+        }
+    *n_options = k;
+    return BFJ_OK;
+}
+
+int bfj_set_tile_rows(bfj_ctx *ctx, int32_t dtype, int32_t tile_rows)
+{
+    if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
+    if (dtype < BFJ_FP64 || dtype > BFJ_FP8) return fail(BFJ_ERR_INVALID, "unknown dtype %d", dtype);
+    for (int kc = 0; kc <= 32; ++kc) ctx->tile_rows[dtype][kc] = tile_rows;
+    return BFJ_OK;
+}
+
+int bfj_autotune(bfj_ctx *ctx, int32_t dtype, int32_t d, int64_t n, int32_t *tile_rows_out)
+{
+    if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
+    if (dtype < BFJ_FP64 || dtype > BFJ_FP8) return fail(BFJ_ERR_INVALID, "unknown dtype %d", dtype);
+    if (d < 1 || d > BFJ_MAX_DIM) return fail(BFJ_ERR_UNSUPPORTED, "d = %d outside [1, %d]", d, BFJ_MAX_DIM);
+    if (n < 256) n = 256;
+    HC(hipSetDevice(ctx->device));
+    const int kc = pick_kc(dtype, d);
+    if (!kc) return fail(BFJ_ERR_UNSUPPORTED, "no kernel for dtype %d d %d", dtype, d);
+    // Synthetic uniform rows; a small eps keeps the epilogue on its fast path, as in
+    // the joins the tiles are tuned for.
+    double *x = nullptr;
+    HC(hipMalloc(&x, (size_t)n * d * 8));
+    hipLaunchKernelGGL(k_synth, dim3(1024), dim3(256), 0, ctx->stream, x, (int64_t)n * d, 12345ull);
+    const double eps = 1e-3 * std::sqrt((double)d);
+    int best = 0;
+    double best_ms = 1e300;
+    for (int r : kRowOptions) {
+        if (!bfj_join_has(dtype, kc, r)) continue;
+        double ms_min = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {
+            bfj_stats st;
+            int rc = join_one(ctx, x, n, d, d, &eps, 1, dtype, 0, n, nullptr, nullptr, &st, r);
+            if (rc != BFJ_OK) {
+                (void)hipFree(x);
+                return rc;
+            }
+            ms_min = std::min(ms_min, st.ms_join);
+        }
+        if (ms_min < best_ms) {
+            best_ms = ms_min;
+            best = r;
+        }
+    }
+    (void)hipFree(x);
+    ctx->tile_rows[dtype][kc] = best;
+    if (tile_rows_out) *tile_rows_out = best;
+    return BFJ_OK;
+}
+
+} // extern "C"

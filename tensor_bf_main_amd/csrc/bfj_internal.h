@@ -1,0 +1,143 @@
+// bfj_internal.h -- device-side data structures shared by the prep, join and
+// refine kernels and the host orchestration (bfj_api.hip).
+//
+// Data layout in HBM for one pass (all datasets of a batch concatenated):
+//   x          caller's FP64 rows, n x ld, row-major (never copied on the device path)
+//   pack       dot-product operands in MFMA fragment order: for every block of 16
+//              rows and every K-chunk, 64 lanes x FRAG_BYTES contiguous, so one
+//              wavefront load of a fragment is a single coalesced 1-2 KiB read
+//   nq[row]    0.5 * ||xhat_j||^2 (FP32 or FP64), +BIG for padding rows
+//   gq[row]    0.5 * (H_i - N_i) rounded up; -BIG for rows that are not queries
+//   ov[row][e] certain-out / certain-in thresholds on Z for each epsilon
+//   hmax[row]  H_i (FP64), used to report the approximation error of refined pairs
+//   cnt[e][row] per-row hit counts, self included (uint32)
+//   band list  (i, j, z, mask) records of pairs whose band straddles a threshold
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BFJ_MAXEPS 8
+#define BFJ_WAVE 64
+#define BFJ_BLOCK 256
+
+typedef _Float16 bfj_h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bfj_b8 __attribute__((ext_vector_type(8)));
+typedef float bfj_f4 __attribute__((ext_vector_type(4)));
+typedef double bfj_d2 __attribute__((ext_vector_type(2)));
+typedef double bfj_d4 __attribute__((ext_vector_type(4)));
+typedef int bfj_i8 __attribute__((ext_vector_type(8)));
+
+enum { BFJ_FP64 = 0, BFJ_FP16 = 1, BFJ_BF16 = 2, BFJ_FP8 = 3 };
+
+// One dataset of a pass.  Row-indexed arrays are addressed with row_off + i,
+// where i is the dataset-local row; rows are padded to a multiple of 16.
+struct SetDesc {
+    const double *x;   // FP64 input rows (device)
+    int64_t n;         // points
+    int64_t npad;      // n rounded up to 16
+    int64_t ld;        // row stride of x in doubles
+    int64_t row_begin; // query rows [row_begin, row_end)
+    int64_t row_end;
+    int64_t row_off;   // offset in the row-indexed arrays
+    int64_t pack_off;  // byte offset of this dataset's fragments in pack
+    int64_t ov_off;    // offset (in threshold pairs) into ov
+    int64_t cnt_off;   // offset into cnt; cnt[cnt_off + e * npad + i]
+    int64_t *out;      // device int64 counts [n_eps][row_end-row_begin], self excluded (may be NULL)
+    int32_t d;         // true dimension
+    int32_t kc;        // K-chunks after padding
+    int32_t n_eps;
+    int32_t set_id;
+    double T[BFJ_MAXEPS]; // input-space thresholds: hit iff d2 <= T
+};
+
+// Per-dataset device scalars produced by the prep kernels.
+struct SetScalars {
+    double sigma;          // power-of-two scale: kernel space = (x - mu) / sigma
+    unsigned long long maxabs_bits; // max |x - mu| (non-negative double as bits)
+    unsigned long long qmax_bits;   // max quantisation error radius (kernel space)
+    unsigned long long nmax_bits;   // max ||xhat||^2 (kernel space)
+    unsigned long long err_bits;    // max |d2_approx - d2_fp64| over refined pairs (input space)
+    unsigned long long halfband_bits; // max (H - L) / 2 (input space)
+    unsigned long long total[BFJ_MAXEPS]; // totals, self included
+};
+
+// A pair whose certain-in / certain-out tests were both inconclusive.
+struct BandEntry {
+    uint32_t i;     // dataset-local query row
+    uint32_t j;     // dataset-local candidate row
+    float z;        // Z value of the pair from the join kernel (kernel space)
+    uint32_t meta;  // bits 0..7: eps mask; bits 16..31: set index
+};
+
+// A join work item: one strip of R query rows of one dataset.
+struct WorkItem {
+    int32_t set;
+    int32_t rb0; // first 16-row block of the strip
+};
+
+struct JoinArgs {
+    const SetDesc *sets;
+    const WorkItem *items;
+    int32_t n_items;
+    uint32_t *item_counter; // persistent-kernel queue head (zeroed before launch)
+    const unsigned char *pack;
+    const void *nq;          // float or double per row
+    const void *gq;          // float or double per row
+    const void *ov;          // float2 or double2 per (row, eps)
+    const uint8_t *scales;   // FP8 per-row E8M0 scale bytes (NULL otherwise)
+    uint32_t *cnt;
+    BandEntry *band;
+    uint32_t *band_count;
+    uint32_t band_capacity;
+};
+
+// Host-side launchers (one translation unit per dtype; see join_*.hip).
+struct JoinLaunch {
+    int dtype;
+    int kc;
+    int rows;           // R: query rows per wavefront
+    int grid_blocks;    // 0: one wavefront per item; >0: persistent grid
+    int lds_bytes;      // dynamic LDS for per-row counts
+};
+
+int bfj_join_launch(const JoinLaunch &L, const JoinArgs &a, hipStream_t s);
+bool bfj_join_has(int dtype, int kc, int rows);
+int bfj_join_max_blocks_per_cu(int dtype, int kc, int rows, int lds_bytes);
+
+// Fragment geometry per dtype: K values per chunk and bytes per lane per chunk.
+__host__ __device__ inline int bfj_kchunk(int dtype) { return dtype == BFJ_FP64 ? 8 : (dtype == BFJ_FP8 ? 128 : 32); }
+__host__ __device__ inline int bfj_frag_bytes(int dtype) { return dtype == BFJ_FP8 ? 32 : 16; }
+__host__ __device__ inline int bfj_acc_bytes(int dtype) { return dtype == BFJ_FP64 ? 8 : 4; }
+
+// Prep / refine / finalize launchers (prep.hip, refine.hip).
+struct PassArgs {
+    const SetDesc *sets;       // device
+    SetScalars *scal;          // device, one per set
+    int32_t n_sets;
+    int32_t dtype;
+    const int32_t *blk_set;    // block -> set table for row-parallel kernels
+    const int64_t *blk_row0;   // block -> first dataset-local row
+    int32_t n_row_blocks;      // blocks in those tables (64 rows each)
+    const int32_t *set_blk0;   // set -> its first block in the tables
+    const int32_t *set_nblk;   // set -> number of its blocks
+    double *colsum;            // [row_blocks][BFJ_MAXDIM] partial column sums
+    double *mu;                // [n_sets][BFJ_MAXDIM]
+    unsigned char *pack;
+    void *nq, *gq, *ov;
+    double *hmax;
+    double *qerr, *nrm;        // per row
+    uint8_t *scales;
+    uint32_t *cnt;
+    int64_t cnt_len;
+    BandEntry *band;
+    uint32_t *band_count;
+    uint32_t band_capacity;
+};
+
+#define BFJ_MAXDIM 256
+#define BFJ_PREP_ROWS 64   // rows per block in row-parallel prep kernels
+
+void bfj_launch_zero(void *p, size_t bytes, hipStream_t s);
+void bfj_launch_prep(const PassArgs &a, hipStream_t s);
+void bfj_launch_refine(const PassArgs &a, hipStream_t s);
+void bfj_launch_finalize(const PassArgs &a, hipStream_t s);

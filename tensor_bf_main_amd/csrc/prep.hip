@@ -1,0 +1,409 @@
+// prep.hip -- per-pass preparation kernels.
+//
+// Replaces convertDataset (kernel_join.cu:44-54: cast FP64 rows to the compute
+// type) and preComputedSquaredCoordinates (kernel_join.cu:57-88: squared norms
+// per row chunk).  Here the rows are first centred on the column means (the
+// distances do not change, the magnitudes that set every rounding error do),
+// scaled by a power of two, cast, and written straight into MFMA fragment
+// order.  For every row the kernel also measures, in FP64, how far the cast row
+// lies from the exact centred row; those radii and the norms set the
+// per-row error band that makes the low-precision decisions rigorous.
+//
+// Band derivation (kernel space; the join kernel computes
+//   Z_ij = MFMA(C0 = g_i - n_j, xhat_i . xhat_j)  ~=  0.5 * (H_i - S_ij),
+//   S_ij = ||xhat_i - xhat_j||^2):
+//   * quantisation: | ||xhat_i - xhat_j|| - ||y_i - y_j|| | <= q_i + q_j,
+//     q_i >= ||y_i - xhat_i|| measured per row, y the exact centred row.
+//   * oracle rounding: the FP64 direct form differs from the exact squared
+//     distance by at most rho = gamma_{d+2}(2^-53) relatively.
+//   * accumulation: |Z - (g_i - 0.5 N_j + xhat_i . xhat_j)| <= E_i / 2 with
+//     E_i = 2 (1.01 K' + 1) u (|g_i| + N_max + sqrt(N_i N_max)), the standard
+//     bound for K' = 2K + 8 round-to-nearest additions in any order at unit
+//     roundoff u (2^-24 for FP32 accumulation, 2^-53 for FP64), plus the
+//     rounding of C0 and of the FP32 norms.
+//   Certain-out for eps e:  Z < o_ie = g_i + 0.5 (N_i - E_i - H_e)  (rounded down)
+//   Certain-in  for eps e:  Z >= v_ie = g_i + 0.5 (N_i + E_i - L_e) (rounded up)
+//   with H_e = (sqrt(T_e/(1-rho)) + r_i)^2, L_e = (sqrt(T_e/(1+rho)) - r_i)^2,
+//   r_i = q_i + q_max, and g_i = 0.5 (H_max + E_i - N_i) rounded up, which
+//   makes o_i,max >= 0: Z < 0 is certainly out for every eps.
+#include "bfj_internal.h"
+#include <math.h>
+#include <float.h>
+
+__global__ void k_zero(uint4 *p, size_t n16, unsigned char *tail, size_t ntail)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+    if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+void bfj_launch_zero(void *p, size_t bytes, hipStream_t s)
+{
+    if (!bytes) return;
+    size_t n16 = bytes / 16, ntail = bytes % 16;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, (uint4 *)p, n16,
+                       (unsigned char *)p + n16 * 16, ntail);
+}
+
+__device__ inline void atomic_max_nonneg(unsigned long long *addr, double v)
+{
+    if (!(v >= 0.0)) return;
+    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ inline double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// ---- column means -----------------------------------------------------------
+// Partial sums over 64-row blocks, then an in-order sum per dataset: the mean is
+// the same bits on every run.
+__global__ void k_colsum(PassArgs a)
+{
+    const int b = blockIdx.x;
+    const SetDesc &s = a.sets[a.blk_set[b]];
+    const int64_t r0 = a.blk_row0[b];
+    for (int k = threadIdx.x; k < s.d; k += blockDim.x) {
+        double acc = 0.0;
+        for (int64_t i = r0; i < r0 + BFJ_PREP_ROWS && i < s.n; ++i) acc += s.x[i * s.ld + k];
+        a.colsum[(int64_t)b * BFJ_MAXDIM + k] = acc;
+    }
+}
+
+__global__ void k_colmean(PassArgs a)
+{
+    const int32_t *set_blk0 = a.set_blk0, *set_nblk = a.set_nblk;
+    const int si = blockIdx.x;
+    const SetDesc &s = a.sets[si];
+    for (int k = threadIdx.x; k < BFJ_MAXDIM; k += blockDim.x) {
+        double acc = 0.0;
+        if (k < s.d)
+            for (int b = 0; b < set_nblk[si]; ++b) acc += a.colsum[(int64_t)(set_blk0[si] + b) * BFJ_MAXDIM + k];
+        a.mu[(int64_t)si * BFJ_MAXDIM + k] = (s.n > 0 && k < s.d) ? acc / (double)s.n : 0.0;
+    }
+}
+
+// ---- scale ------------------------------------------------------------------
+__global__ void k_maxabs(PassArgs a)
+{
+    __shared__ double red[BFJ_BLOCK / BFJ_WAVE];
+    const int b = blockIdx.x;
+    const int si = a.blk_set[b];
+    const SetDesc &s = a.sets[si];
+    const double *mu = a.mu + (int64_t)si * BFJ_MAXDIM;
+    const int64_t r0 = a.blk_row0[b];
+    double m = 0.0;
+    for (int64_t t = threadIdx.x; t < (int64_t)BFJ_PREP_ROWS * s.d; t += blockDim.x) {
+        const int64_t i = r0 + t / s.d;
+        const int k = (int)(t % s.d);
+        if (i < s.n) m = fmax(m, fabs(s.x[i * s.ld + k] - mu[k]));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BFJ_BLOCK / BFJ_WAVE; ++w) m = fmax(m, red[w]);
+        atomic_max_nonneg(&a.scal[si].maxabs_bits, red[0] > m ? red[0] : m);
+    }
+}
+
+// sigma = 2^ceil(log2(max|x - mu|)) so the centred, scaled rows lie in [-1, 1].
+// FP64 keeps sigma = 1 (no cast, nothing to protect).
+__global__ void k_sigma(PassArgs a)
+{
+    const int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si >= a.n_sets) return;
+    const double m = __longlong_as_double((long long)a.scal[si].maxabs_bits);
+    double sigma = 1.0;
+    if (a.dtype != BFJ_FP64 && m > 0.0 && isfinite(m)) {
+        int e;
+        frexp(m, &e); // m in [2^(e-1), 2^e)
+        sigma = ldexp(1.0, e);
+    }
+    a.scal[si].sigma = sigma;
+}
+
+// ---- cast + pack + norms + quantisation radius ---------------------------------
+__device__ inline uint8_t e4m3_encode(double v)
+{
+    // OCP FP8 e4m3 (bias 7, max 448, subnormals m * 2^-9), round to nearest even.
+    const uint8_t sgn = v < 0.0 ? 0x80 : 0x00;
+    double a = fabs(v);
+    if (a > 448.0) a = 448.0;
+    if (a == 0.0) return sgn;
+    int e;
+    frexp(a, &e);
+    int E = e - 1; // a in [2^E, 2^(E+1))
+    if (E < -6) {
+        const double m = rint(ldexp(a, 9));
+        if (m >= 8.0) return sgn | (uint8_t)(1 << 3);
+        return sgn | (uint8_t)m;
+    }
+    double m = rint(ldexp(a, 3 - E)); // in [8, 16]
+    if (m >= 16.0) { E += 1; m = 8.0; }
+    if (E > 8 || (E == 8 && m > 14.0)) { E = 8; m = 14.0; }
+    return sgn | (uint8_t)(((E + 7) << 3) | ((int)m - 8));
+}
+
+__device__ inline double e4m3_decode(uint8_t b)
+{
+    const double sg = (b & 0x80) ? -1.0 : 1.0;
+    const int ef = (b >> 3) & 15, m = b & 7;
+    if (ef == 0) return sg * ldexp((double)m, -9);
+    return sg * ldexp((double)(8 + m), ef - 10);
+}
+
+// One wavefront per row; 4 wavefronts x 16 rows per block.
+__global__ __launch_bounds__(BFJ_BLOCK) void k_quantize(PassArgs a)
+{
+    __shared__ double redq[BFJ_BLOCK / BFJ_WAVE], redn[BFJ_BLOCK / BFJ_WAVE];
+    const int b = blockIdx.x;
+    const int si = a.blk_set[b];
+    const SetDesc &s = a.sets[si];
+    const double *mu = a.mu + (int64_t)si * BFJ_MAXDIM;
+    const double sigma = a.scal[si].sigma;
+    const double inv_sigma = 1.0 / sigma; // exact: sigma is a power of two
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int KCH = bfj_kchunk(a.dtype);
+    const int FB = bfj_frag_bytes(a.dtype);
+    const int ES = a.dtype == BFJ_FP64 ? 8 : (a.dtype == BFJ_FP8 ? 1 : 2);
+    const int EPL = FB / ES; // elements per lane per chunk
+    const int Kpad = s.kc * KCH;
+    double qmax = 0.0, nmax = 0.0;
+    for (int rr = 0; rr < BFJ_PREP_ROWS / 4; ++rr) {
+        const int64_t i = a.blk_row0[b] + wave * (BFJ_PREP_ROWS / 4) + rr;
+        if (i >= s.npad) break;
+        const bool real = i < s.n;
+        // Centred, scaled row values for this lane's k = lane + 64 t.
+        double y[BFJ_MAXDIM / BFJ_WAVE];
+        double rmax = 0.0;
+#pragma unroll
+        for (int t = 0; t < BFJ_MAXDIM / BFJ_WAVE; ++t) {
+            const int k = lane + BFJ_WAVE * t;
+            double v = 0.0;
+            if (real && k < s.d) v = (s.x[i * s.ld + k] - mu[k]) * inv_sigma;
+            y[t] = v;
+            rmax = fmax(rmax, fabs(v));
+        }
+        int scale_byte = 127;
+        double rs = 1.0;
+        if (a.dtype == BFJ_FP8) {
+            rmax = wave_max(rmax);
+            if (rmax > 0.0) {
+                int e;
+                frexp(rmax / 448.0, &e); // rmax/448 in [2^(e-1), 2^e)
+                int ex = e;
+                if (ldexp(1.0, e - 1) == rmax / 448.0) ex = e - 1;
+                if (ex < -127) ex = -127;
+                if (ex > 127) ex = 127;
+                scale_byte = 127 + ex;
+                rs = ldexp(1.0, ex);
+            }
+        }
+        double sq_e = 0.0, sq_n = 0.0, sq_y = 0.0;
+        unsigned char *base = a.pack + s.pack_off + (i / 16) * (int64_t)s.kc * BFJ_WAVE * FB;
+#pragma unroll
+        for (int t = 0; t < BFJ_MAXDIM / BFJ_WAVE; ++t) {
+            const int k = lane + BFJ_WAVE * t;
+            if (k >= Kpad) break;
+            const double v = y[t];
+            double xh;
+            const int c = k / KCH, kl = k % KCH, g = kl / EPL, vv = kl % EPL;
+            unsigned char *dst = base + ((int64_t)c * BFJ_WAVE + (i % 16) + 16 * g) * FB + vv * ES;
+            if (a.dtype == BFJ_FP64) {
+                xh = v;
+                *(double *)dst = xh;
+            } else if (a.dtype == BFJ_FP16) {
+                const _Float16 h = (_Float16)v;
+                xh = (double)h;
+                *(_Float16 *)dst = h;
+            } else if (a.dtype == BFJ_BF16) {
+                const __bf16 h = (__bf16)(float)v;
+                xh = (double)(float)h;
+                *(__bf16 *)dst = h;
+            } else {
+                const uint8_t q = e4m3_encode(v / rs);
+                xh = e4m3_decode(q) * rs;
+                *(uint8_t *)dst = q;
+            }
+            const double de = v - xh;
+            sq_e = fma(de, de, sq_e);
+            sq_n = fma(xh, xh, sq_n);
+            sq_y = fma(v, v, sq_y);
+        }
+        sq_e = wave_sum(sq_e);
+        sq_n = wave_sum(sq_n);
+        sq_y = wave_sum(sq_y);
+        if (lane == 0) {
+            const int64_t row = s.row_off + i;
+            // Radius: measured cast error, plus the FP64 rounding of (x - mu), inflated
+            // to cover the FP64 evaluation of the sums.
+            const double q = real ? (sqrt(sq_e) + 2.3e-16 * sqrt(sq_y)) * (1.0 + 1e-9) : 0.0;
+            a.qerr[row] = q;
+            a.nrm[row] = sq_n;
+            if (a.dtype == BFJ_FP8) a.scales[row] = (uint8_t)scale_byte;
+            if (a.dtype == BFJ_FP64) ((double *)a.nq)[row] = real ? 0.5 * sq_n : 1e300;
+            else ((float *)a.nq)[row] = real ? 0.5f * (float)sq_n : 1e30f;
+            if (real) {
+                qmax = fmax(qmax, q);
+                nmax = fmax(nmax, sq_n);
+            }
+        }
+    }
+    if (lane == 0) {
+        redq[wave] = qmax;
+        redn[wave] = nmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BFJ_BLOCK / BFJ_WAVE; ++w) {
+            qmax = fmax(qmax, redq[w]);
+            nmax = fmax(nmax, redn[w]);
+        }
+        atomic_max_nonneg(&a.scal[si].qmax_bits, qmax);
+        atomic_max_nonneg(&a.scal[si].nmax_bits, nmax);
+    }
+}
+
+// ---- thresholds -----------------------------------------------------------------
+__device__ inline float f32_up(double v)
+{
+    if (isnan(v)) return __int_as_float(0x7fc00000);
+    if (v >= 3.0e38) return INFINITY;
+    if (v <= -3.0e38) return -3.0e38f;
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+__device__ inline float f32_down(double v)
+{
+    if (isnan(v)) return __int_as_float(0x7fc00000);
+    if (v <= -3.0e38) return -INFINITY;
+    if (v >= 3.0e38) return 3.0e38f;
+    float f = (float)v;
+    if ((double)f > v) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+// One thread per row.
+__global__ void k_thresholds(PassArgs a)
+{
+    __shared__ double redh[BFJ_PREP_ROWS];
+    const int b = blockIdx.x;
+    const int si = a.blk_set[b];
+    const SetDesc &s = a.sets[si];
+    const SetScalars &sc = a.scal[si];
+    const int64_t i = a.blk_row0[b] + threadIdx.x;
+    double halfband = 0.0;
+    if (i < s.npad) {
+        const int64_t row = s.row_off + i;
+        const bool fp64 = a.dtype == BFJ_FP64;
+        const bool query = i < s.n && i >= s.row_begin && i < s.row_end;
+        const double sigma2 = sc.sigma * sc.sigma;
+        const double qmax = __longlong_as_double((long long)sc.qmax_bits);
+        const double nmax = __longlong_as_double((long long)sc.nmax_bits);
+        const double Ni = a.nrm[row];
+        const double r = a.qerr[row] + qmax;
+        const int K = s.kc * bfj_kchunk(a.dtype);
+        const double u = fp64 ? 0x1p-53 : 0x1p-24;
+        const double rho = 1.01 * (s.d + 2) * 0x1p-53;
+        const double u64 = 0x1p-53;
+        double Hc[BFJ_MAXEPS], Lc[BFJ_MAXEPS];
+        double Hmax = -INFINITY;
+        bool any_inf = false;
+        for (int e = 0; e < s.n_eps; ++e) {
+            const double T = s.T[e] / sigma2; // kernel space (exact power-of-two scaling)
+            if (T < 0.0) {
+                Hc[e] = -INFINITY;
+                Lc[e] = -INFINITY;
+            } else if (isinf(T)) {
+                Hc[e] = INFINITY;
+                Lc[e] = INFINITY;
+                any_inf = true;
+            } else {
+                const double slo = sqrt(T / (1.0 + rho)) * (1.0 - 0x1p-50);
+                const double shi = sqrt(T / (1.0 - rho)) * (1.0 + 0x1p-50);
+                Lc[e] = slo > r ? (slo - r) * (slo - r) * (1.0 - 0x1p-49) : -INFINITY;
+                Hc[e] = (shi + r) * (shi + r) * (1.0 + 0x1p-49);
+                if (query) halfband = fmax(halfband, 0.5 * (Hc[e] - (Lc[e] > 0.0 ? Lc[e] : 0.0)) * sigma2);
+            }
+            Hmax = fmax(Hmax, Hc[e]);
+        }
+        double g; // FP64 value of the per-row initial term actually stored
+        double E = 0.0;
+        if (!query || Hmax == -INFINITY) {
+            g = fp64 ? -1e300 : -1e30;
+        } else if (any_inf || Hmax > 1e30) {
+            g = fp64 ? 1e300 : 1e30;
+        } else {
+            // Accumulation bound; |g| is bounded from the previous estimate.
+            const double Kp = 2.0 * K + 8.0;
+            const double coef = 2.0 * (1.01 * Kp + 1.0) * u;
+            const double cross = sqrt(Ni * nmax);
+            double gabs = 0.5 * fabs(Hmax - Ni);
+            for (int it = 0; it < 2; ++it) {
+                E = coef * (gabs + nmax + cross) + 4.0 * (K + 4) * u64 * (Ni + nmax + gabs) + K * 0x1p-120;
+                gabs = 0.5 * fabs(Hmax - Ni) + E;
+            }
+            E *= 1.01;
+            const double gv = 0.5 * (Hmax + E - Ni) * (1.0 + 0x1p-50) + (Hmax + E - Ni < 0.0 ? -0x1p-1000 : 0x1p-1000);
+            g = fp64 ? gv : (double)f32_up(gv);
+        }
+        if (fp64) ((double *)a.gq)[row] = g;
+        else ((float *)a.gq)[row] = (float)g;
+        a.hmax[row] = 2.0 * g + Ni;
+        for (int e = 0; e < s.n_eps; ++e) {
+            double o, v;
+            if (!query || Hc[e] == -INFINITY) {
+                o = INFINITY; // everything out
+                v = INFINITY;
+            } else if (Hc[e] == INFINITY) {
+                o = -INFINITY;
+                v = -INFINITY; // everything in
+            } else {
+                o = g + 0.5 * (Ni - E - Hc[e]);
+                v = g + 0.5 * (Ni + E - Lc[e]); // +inf when Lc = -inf
+                o = o - fabs(o) * 0x1p-50;
+                v = v + fabs(v) * 0x1p-50;
+                if (Hc[e] == Hmax && o < 0.0) o = 0.0; // exact by construction of g
+            }
+            if (fp64) {
+                double2 *ov = (double2 *)a.ov + s.ov_off + i * s.n_eps + e;
+                *ov = make_double2(o, v);
+            } else {
+                float2 *ov = (float2 *)a.ov + s.ov_off + i * s.n_eps + e;
+                *ov = make_float2(f32_down(o), f32_up(v));
+            }
+        }
+    }
+    redh[threadIdx.x] = halfband;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int t = 0; t < BFJ_PREP_ROWS; ++t) m = fmax(m, redh[t]);
+        atomic_max_nonneg(&a.scal[si].halfband_bits, m);
+    }
+}
+
+void bfj_launch_prep(const PassArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_colsum, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_colmean, dim3(a.n_sets), dim3(BFJ_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_maxabs, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_sigma, dim3((a.n_sets + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_quantize, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_thresholds, dim3(a.n_row_blocks), dim3(BFJ_PREP_ROWS), 0, st, a);
+}

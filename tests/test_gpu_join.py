@@ -1,0 +1,197 @@
+"""GPU parity tests: the HIP join (through the C ABI) against the CPU FP64 oracle.
+
+The bar is bit-exact per-point counts and totals for every dtype: the
+low-precision paths decide only the pairs whose rigorous error band excludes
+the threshold and re-evaluate the rest in FP64 on the GPU.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import tensor_bf_main_amd as bfj
+from oracle import oracle
+from tensor_bf_main_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+DTYPES = ["fp64", "fp16", "bf16", "fp8"]
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _check(ctx, x, eps, dtype, row_begin=0, row_end=None):
+    n = x.shape[0]
+    re_ = n if row_end is None else row_end
+    res = ctx.self_join(x, eps, dtype=dtype, row_begin=row_begin, row_end=re_)
+    oc, ot = oracle.join(x, eps, row_begin, re_)
+    np.testing.assert_array_equal(res.counts, oc, err_msg=f"per-point counts differ ({dtype})")
+    np.testing.assert_array_equal(res.totals, ot, err_msg=f"totals differ ({dtype})")
+    return res
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_reference_test_files(ctx, dtype):
+    gold = json.load(open(os.path.join(GOLDEN, "expected_counts.json")))
+    for name in ("test100.txt", "test128.txt"):
+        rows = oracle.parse_text(open(os.path.join(GOLDEN, name)).read(), 18)
+        g = gold[name]
+        res = ctx.self_join(rows, g["eps"], dtype=dtype)
+        assert [int(t) for t in res.totals] == g["totals"]
+        np.testing.assert_array_equal(res.counts, np.asarray(g["counts"], dtype=np.int64))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("n,d", [(1, 1), (2, 3), (17, 5), (300, 18), (1000, 32), (2000, 128), (1500, 200),
+                                 (1200, 256)])
+def test_uniform_small(ctx, dtype, n, d):
+    x = wl.uniform(n, d, 7 + n + d)
+    eps = wl.eps_quantile(x, [0.0, 0.01, 0.1, 0.5], 200, 3) if n > 1 else [0.0, 1.0]
+    _check(ctx, x, eps, dtype)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_workload1_full(ctx, dtype):
+    x, eps = wl.config(1)
+    res = _check(ctx, x, eps, dtype)
+    assert int(res.totals[0]) == int(res.counts[0].sum()) + x.shape[0]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_duplicates_and_eps_edges(ctx, dtype):
+    rng = np.random.default_rng(11)
+    base = rng.random((200, 24))
+    x = np.concatenate([base, base[:50], base[:10]])  # rows 0..9 appear 3x, 10..49 twice
+    eps = [0.0, -1.0, float("inf"), 1e300, 1e-300, 0.25]
+    res = _check(ctx, x, eps, dtype)
+    n = x.shape[0]
+    # eps = 0: exactly the exact duplicates; eps < 0: nothing; inf / huge: everything.
+    assert int(res.totals[1]) == 0 and res.counts[1].sum() == -n
+    assert int(res.totals[2]) == n * n and int(res.totals[3]) == n * n
+    assert int(res.totals[0]) == n + 2 * (10 * 3 + 40)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_exact_boundary_pairs(ctx, dtype):
+    """eps set to the exact FP64 distance of real pairs: those pairs must count."""
+    x = wl.gmm(3000, 64, 8, 0.05, 21)
+    rng = np.random.default_rng(5)
+    i = rng.integers(0, 3000, 6)
+    j = rng.integers(0, 3000, 6)
+    d2 = oracle.pair_d2(x, i, j)
+    eps = [float(np.sqrt(v)) for v in d2]
+    # sqrt is correctly rounded: sqrt(d2) <= eps holds with equality for these pairs.
+    _check(ctx, x, eps, dtype)
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_row_range_and_stride(ctx, dtype):
+    x0 = wl.uniform(1700, 40, 9)
+    wide = np.zeros((1700, 57))
+    wide[:, :40] = x0
+    x = wide[:, :40]  # ld = 57 > d
+    eps = wl.eps_quantile(x0, [0.002, 0.05], 300, 1)
+    _check(ctx, x, eps, dtype, row_begin=333, row_end=1201)
+
+
+@pytest.mark.parametrize("dtype", ["fp64", "fp16"])
+def test_workload2_reduced(ctx, dtype):
+    x, eps = wl.config(2, n=20000)
+    res = _check(ctx, x, eps, dtype)
+    assert res.stats["band_pairs"] >= 0
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_workload3_reduced(ctx, dtype):
+    x, eps = wl.config(3, n=30000)
+    res = _check(ctx, x, eps, dtype)
+    if dtype == "fp16":
+        assert res.stats["max_band_halfwidth"] > res.stats["max_abs_d2_err"]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_workload4_reduced(ctx, dtype):
+    x, eps = wl.config(4, n=12000)
+    _check(ctx, x, eps, dtype)
+
+
+def test_multi_eps_equals_single(ctx):
+    x, _ = wl.config(3, n=8000)
+    eps = wl.eps_quantile(x, [0.0005, 0.001, 0.002, 0.005, 0.01, 0.02, 0.05, 0.1], 400, 2)
+    multi = ctx.self_join(x, eps, dtype="fp16")
+    for k, e in enumerate(eps):
+        single = ctx.self_join(x, e, dtype="fp16")
+        np.testing.assert_array_equal(multi.counts[k], single.counts[0])
+        assert multi.totals[k] == single.totals[0]
+
+
+@pytest.mark.parametrize("dtype", ["fp16", "fp64"])
+def test_batch_mixed_dims(ctx, dtype):
+    sets = wl.batch(n_sets=12, seed=77, n_min=100, n_max=3000)
+    out, stats = ctx.self_join_batch(sets, dtype=dtype)
+    for s, (counts, totals) in zip(sets, out):
+        oc, ot = oracle.join(s["data"], s["eps"])
+        np.testing.assert_array_equal(counts, oc)
+        np.testing.assert_array_equal(totals, ot)
+    assert stats["pairs"] == sum(s["data"].shape[0] ** 2 for s in sets)
+
+
+def test_plan_graph_replays(ctx):
+    import torch
+
+    sets = wl.batch(n_sets=6, seed=78, n_min=200, n_max=2500)
+    dev = [{"data": torch.from_numpy(s["data"]).cuda(), "eps": s["eps"],
+            "counts": torch.zeros((len(s["eps"]), s["data"].shape[0]), dtype=torch.int64, device="cuda")}
+           for s in sets]
+    plan = ctx.plan(dev, dtype="fp16")
+    for rep in range(3):
+        plan.run()
+        torch.cuda.synchronize()
+        for s, dv, tot in zip(sets, dev, plan.totals()):
+            oc, ot = oracle.join(s["data"], s["eps"])
+            np.testing.assert_array_equal(dv["counts"].cpu().numpy(), oc, err_msg=f"replay {rep}")
+            np.testing.assert_array_equal(tot, ot)
+        # eager work on the same context between replays
+        ctx.self_join(sets[0]["data"], sets[0]["eps"], dtype="fp64")
+    plan.close()
+
+
+def test_device_resident_path(ctx):
+    import torch
+
+    x, eps = wl.config(1, n=5000)
+    xt = torch.from_numpy(x).cuda()
+    cnt = torch.zeros((1, 5000), dtype=torch.int64, device="cuda")
+    res = ctx.self_join_device(xt, eps, dtype="fp16", counts=cnt)
+    oc, ot = oracle.join(x, eps)
+    np.testing.assert_array_equal(cnt.cpu().numpy(), oc)
+    np.testing.assert_array_equal(res.totals, ot)
+
+
+def test_autotune_then_exact(ctx):
+    r = ctx.autotune("fp16", 128, n=8192)
+    assert r in bfj.tile_rows_options("fp16", 128)
+    x, eps = wl.config(3, n=6000)
+    _check(ctx, x, eps, "fp16")
+    ctx.set_tile_rows("fp16", 0)
+
+
+@pytest.mark.slow
+def test_workload3_full_fp16_vs_fp64(ctx):
+    """n = 1M, d = 128: FP16+refine and FP64 counts agree exactly; 1000 seeded rows
+    match the oracle; the ordered-pair relation is symmetric (even total)."""
+    import torch
+
+    x, eps = wl.config(3)
+    n = x.shape[0]
+    xt = torch.from_numpy(x).cuda()
+    c16 = torch.zeros((1, n), dtype=torch.int64, device="cuda")
+    c64 = torch.zeros((1, n), dtype=torch.int64, device="cuda")
+    r16 = ctx.self_join_device(xt, eps, dtype="fp16", counts=c16)
+    r64 = ctx.self_join_device(xt, eps, dtype="fp64", counts=c64)
+    a16, a64 = c16.cpu().numpy()[0], c64.cpu().numpy()[0]
+    np.testing.assert_array_equal(a16, a64)
+    assert r16.totals[0] == r64.totals[0]
+    assert int(a16.sum()) % 2 == 0
+    rows = np.random.default_rng(4242).choice(n, 1000, replace=False)
+    oc = oracle.join_rows(x, rows, eps)[0]
+    np.testing.assert_array_equal(a16[rows], oc)
