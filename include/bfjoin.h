@@ -125,6 +125,14 @@ int bfj_plan_create(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32
 int bfj_plan_run(bfj_plan *plan, bfj_stats *stats);
 int bfj_plan_destroy(bfj_plan *plan);
 
+/* Device buffers on the context's device (for callers without a GPU allocator
+ * of their own; any device pointer from the same HIP runtime works too).
+ * Copies run on the context stream and return when complete. */
+int bfj_buffer_alloc(bfj_ctx *ctx, uint64_t bytes, void **d_ptr);
+int bfj_buffer_free(bfj_ctx *ctx, void *d_ptr);
+int bfj_buffer_upload(bfj_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
+int bfj_buffer_download(bfj_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
+
 /* Tile choice.  bfj_autotune times every built tile shape for (dtype, d) on a
  * synthetic problem of n points and keeps the fastest in the context; later
  * joins use it.  bfj_set_tile_rows forces a shape (0 = heuristic). */

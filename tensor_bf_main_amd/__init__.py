@@ -101,6 +101,10 @@ def load_library(path: str = LIB_PATH):
         "bfj_autotune": (ctypes.c_int, [vp, i32, i32, i64, ctypes.POINTER(i32)]),
         "bfj_set_tile_rows": (ctypes.c_int, [vp, i32, i32]),
         "bfj_tile_rows_options": (ctypes.c_int, [i32, i32, ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]),
+        "bfj_buffer_alloc": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(vp)]),
+        "bfj_buffer_free": (ctypes.c_int, [vp, vp]),
+        "bfj_buffer_upload": (ctypes.c_int, [vp, vp, vp, ctypes.c_uint64]),
+        "bfj_buffer_download": (ctypes.c_int, [vp, vp, vp, ctypes.c_uint64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -143,6 +147,57 @@ def tile_rows_options(dtype, d: int) -> list[int]:
     k = ctypes.c_int32(0)
     _check(lib.bfj_tile_rows_options(_dtype_id(dtype), d, opts, 8, ctypes.byref(k)))
     return [opts[i] for i in range(min(k.value, 8))]
+
+
+class DeviceArray:
+    """A device buffer owned by a :class:`Context` (row-major, C order)."""
+
+    def __init__(self, ctx: "Context", shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = ctypes.c_void_p()
+        _check(ctx.lib.bfj_buffer_alloc(ctx._h, self.nbytes, ctypes.byref(h)))
+        self._ptr = h
+
+    def data_ptr(self) -> int:
+        return self._ptr.value or 0
+
+    def stride(self, dim: int) -> int:
+        return int(np.prod(self.shape[dim + 1:])) if dim < len(self.shape) else 1
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=self.dtype)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        _check(self.ctx.lib.bfj_buffer_upload(self.ctx._h, self._ptr, a.ctypes.data, self.nbytes))
+        return self
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(self.ctx.lib.bfj_buffer_download(self.ctx._h, out.ctypes.data, self._ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "_ptr", None) and self._ptr.value and getattr(self.ctx, "_h", None):
+            self.ctx.lib.bfj_buffer_free(self.ctx._h, self._ptr)
+        self._ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _dev_shape(x):
+    """(n, d, ld, ptr) of a device matrix: a DeviceArray or any object with
+    data_ptr()/shape/stride() (e.g. a torch tensor from the same HIP runtime)."""
+    n, d = int(x.shape[0]), int(x.shape[1])
+    if x.stride(1) != 1:
+        raise ValueError("device rows need unit column stride")
+    return n, d, (int(x.stride(0)) if n > 1 else d), int(x.data_ptr())
 
 
 @dataclass
@@ -203,30 +258,34 @@ class Context:
             tot.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(st)))
         return JoinResult(out, tot, st.as_dict())
 
+    def to_device(self, host, dtype=np.float64) -> DeviceArray:
+        a = np.ascontiguousarray(host, dtype=dtype)
+        return DeviceArray(self, a.shape, a.dtype).upload(a)
+
+    def empty(self, shape, dtype=np.float64) -> DeviceArray:
+        return DeviceArray(self, shape, dtype)
+
     def self_join_device(self, x, eps, dtype="fp16", row_begin: int = 0, row_end: int | None = None,
                          counts=None) -> JoinResult:
-        """Join a device-resident torch float64 tensor (n, d) with unit column stride.
+        """Join device-resident float64 rows ``x`` (a :class:`DeviceArray` of shape (n, d), or
+        another device matrix with unit column stride).
 
-        ``counts``: optional int64 device tensor of shape (n_eps, rows) that receives
-        the per-point counts."""
-        import torch
-
-        if x.dtype != torch.float64 or x.dim() != 2 or not x.is_cuda or x.stride(1) != 1:
-            raise ValueError("x must be a 2-D float64 CUDA tensor with unit column stride")
-        n, d = x.shape
-        ld = x.stride(0) if n > 1 else d
+        ``counts``: optional int64 device buffer of shape (n_eps, rows) receiving the
+        per-point counts (self excluded)."""
+        if isinstance(x, DeviceArray) and x.dtype != np.float64:
+            raise ValueError("rows must be float64")
+        n, d, ld, ptr = _dev_shape(x)
         e = _eps_array(eps)
         if row_end is None:
             row_end = n
         if counts is not None:
             rows = row_end - row_begin
-            if counts.dtype != torch.int64 or tuple(counts.shape) != (e.size, rows) or not counts.is_contiguous():
-                raise ValueError("counts must be a contiguous int64 device tensor of shape (n_eps, rows)")
+            if tuple(counts.shape) != (e.size, rows):
+                raise ValueError("counts must have shape (n_eps, rows)")
         tot = np.zeros(e.size, dtype=np.uint64)
         st = Stats()
-        torch.cuda.synchronize(x.device)  # producer work on torch's streams must be complete
         _check(self.lib.bfj_self_join_device(
-            self._h, x.data_ptr() if n else None, n, d, ld, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            self._h, ptr if n else None, n, d, ld, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
             e.size, _dtype_id(dtype), row_begin, row_end, counts.data_ptr() if counts is not None else None,
             tot.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(st)))
         return JoinResult(None, tot, st.as_dict())
@@ -239,8 +298,7 @@ class Context:
             data, eps = s["data"], _eps_array(s["eps"])
             tot = np.zeros(eps.size, dtype=np.uint64)
             if device_resident:
-                n, d = data.shape
-                ptr, ld = data.data_ptr(), data.stride(0) if n > 1 else d
+                n, d, ld, ptr = _dev_shape(data)
                 cnt = s.get("counts")
                 cptr = cnt.data_ptr() if cnt is not None else None
             else:
@@ -263,7 +321,8 @@ class Context:
         return [(c, t) for (_, _, t, c) in keep], st.as_dict()
 
     def plan(self, sets, dtype="fp16") -> "Plan":
-        """sets: list of {"data": CUDA float64 tensor, "eps": [...], "counts": optional int64 tensor}."""
+        """sets: list of {"data": device float64 (n, d), "eps": [...], "counts": optional int64
+        device buffer (n_eps, n)}."""
         return Plan(self, sets, dtype)
 
     # -- tiles -----------------------------------------------------------------------------
@@ -280,10 +339,7 @@ class Plan:
     """A batch captured once into a hipGraph; :meth:`run` replays it."""
 
     def __init__(self, ctx: Context, sets, dtype="fp16"):
-        import torch
-
         self.ctx = ctx
-        torch.cuda.synchronize()
         self._arr, self._keep = ctx._datasets(sets, device_resident=True)
         h = ctypes.c_void_p()
         _check(ctx.lib.bfj_plan_create(ctx._h, self._arr, len(sets), _dtype_id(dtype), ctypes.byref(h)))

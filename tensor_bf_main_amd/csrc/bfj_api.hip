@@ -451,6 +451,14 @@ static double bits_to_double(unsigned long long b)
     return d;
 }
 
+// The kernels assume finite rows; a pass over NaN or inf data is reported, not returned.
+static int check_finite(const std::vector<SetScalars> &sc)
+{
+    for (size_t i = 0; i < sc.size(); ++i)
+        if (sc[i].nonfinite) return fail(BFJ_ERR_INVALID, "dataset %zu contains NaN or infinite values", i);
+    return BFJ_OK;
+}
+
 // Run a staged pass to completion, growing the refine list and repeating the
 // pass when it overflowed.  Totals per set land in totals_out[set][eps].
 static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>> *totals_out, bfj_stats *stats,
@@ -465,6 +473,7 @@ static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>
         if (band_count <= ctx->band_capacity) {
             std::vector<SetScalars> sc(p.sets.size());
             HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
+            RC(check_finite(sc));
             if (totals_out) {
                 totals_out->assign(p.sets.size(), {});
                 for (size_t si = 0; si < p.sets.size(); ++si)
@@ -793,6 +802,7 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
         if (band_count <= ctx->band_capacity) {
             std::vector<SetScalars> sc(pl->pass.sets.size());
             HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
+            RC(check_finite(sc));
             for (size_t i = 0; i < pl->sets.size(); ++i)
                 if (pl->sets[i].totals)
                     for (int e = 0; e < pl->sets[i].n_eps; ++e) pl->sets[i].totals[e] = sc[i].total[e];
@@ -834,6 +844,51 @@ int bfj_plan_destroy(bfj_plan *pl)
     if (pl->exec) (void)hipGraphExecDestroy(pl->exec);
     if (pl->graph) (void)hipGraphDestroy(pl->graph);
     delete pl;
+    return BFJ_OK;
+}
+
+// ---- device buffers -------------------------------------------------------------------
+int bfj_buffer_alloc(bfj_ctx *ctx, uint64_t bytes, void **d_ptr)
+{
+    if (!ctx || !d_ptr) return fail(BFJ_ERR_INVALID, "ctx or out-pointer is NULL");
+    *d_ptr = nullptr;
+    HC(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        *d_ptr = nullptr;
+        (void)hipGetLastError();
+        return fail(BFJ_ERR_NOMEM, "hipMalloc(%llu) failed: %s", (unsigned long long)bytes, hipGetErrorString(e));
+    }
+    return BFJ_OK;
+}
+
+int bfj_buffer_free(bfj_ctx *ctx, void *d_ptr)
+{
+    if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
+    if (!d_ptr) return BFJ_OK;
+    HC(hipSetDevice(ctx->device));
+    HC(hipStreamSynchronize(ctx->stream));
+    HC(hipFree(d_ptr));
+    return BFJ_OK;
+}
+
+int bfj_buffer_upload(bfj_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes)
+{
+    if (!ctx || (bytes && (!d_dst || !h_src))) return fail(BFJ_ERR_INVALID, "NULL argument");
+    if (!bytes) return BFJ_OK;
+    HC(hipSetDevice(ctx->device));
+    HC(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    return BFJ_OK;
+}
+
+int bfj_buffer_download(bfj_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes)
+{
+    if (!ctx || (bytes && (!d_src || !h_dst))) return fail(BFJ_ERR_INVALID, "NULL argument");
+    if (!bytes) return BFJ_OK;
+    HC(hipSetDevice(ctx->device));
+    HC(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
     return BFJ_OK;
 }
 

@@ -159,25 +159,39 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][c], B[c], acc[rb], SA[rb], SB);
 
+        // Inputs are NaN-free (the pass rejects non-finite rows), so with
+        // -fno-honor-nans this is a tree of v_max3.  -0.0 >= 0 holds, so a zero Z
+        // of either sign reaches the exact slow-path tests.
         val_t m = acc[0][0];
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) m = acc[rb][r] > m ? acc[rb][r] : m;
+            for (int r = 0; r < 4; ++r) m = fmax(m, acc[rb][r]);
 
         if (__builtin_expect(__ballot(m >= (val_t)0) != 0ull, 0)) {
             // Slow path, taken by the few tiles where some pair is not certainly
-            // out: park the tile in this wavefront's LDS and walk it with a rolled
-            // loop, so the rare path costs neither registers nor code in the fast one.
+            // out: park only the hot 16x16 tiles in this wavefront's LDS (one
+            // 16-byte store per lane each) and walk them with a rolled loop, so
+            // the rare path costs neither registers nor code in the fast one.
+            uint32_t hot = 0;
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) zt[(rb * 4 + r) * BFJ_WAVE + lane] = acc[rb][r];
+            for (int rb = 0; rb < RB; ++rb) {
+                const val_t mr = fmax(fmax(acc[rb][0], acc[rb][1]), fmax(acc[rb][2], acc[rb][3]));
+                if (__ballot(mr >= (val_t)0) != 0ull) {
+                    hot |= 1u << rb;
+                    *reinterpret_cast<acc_t *>(zt + (rb * BFJ_WAVE + lane) * 4) = acc[rb];
+                }
+            }
             const int64_t col = cb * 16 + (lane & 15);
             const bool col_ok = col < s.n;
+            hot = (uint32_t)__builtin_amdgcn_readfirstlane((int)hot);
 #pragma unroll 1
             for (int q = 0; q < RB * 4; ++q) {
-                const val_t z = zt[q * BFJ_WAVE + lane];
+                if (!((hot >> (q >> 2)) & 1u)) {
+                    q |= 3; // skip the rest of this row block
+                    continue;
+                }
+                const val_t z = zt[((q >> 2) * BFJ_WAVE + lane) * 4 + (q & 3)];
                 const bool cand = z >= (val_t)0 && col_ok;
                 if (__ballot(cand) == 0ull) continue;
                 const int li = (q >> 2) * 16 + Tr::out_row(lane, q & 3);

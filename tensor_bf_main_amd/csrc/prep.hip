@@ -105,11 +105,17 @@ __global__ void k_maxabs(PassArgs a)
     const double *mu = a.mu + (int64_t)si * BFJ_MAXDIM;
     const int64_t r0 = a.blk_row0[b];
     double m = 0.0;
+    bool bad = false;
     for (int64_t t = threadIdx.x; t < (int64_t)BFJ_PREP_ROWS * s.d; t += blockDim.x) {
         const int64_t i = r0 + t / s.d;
         const int k = (int)(t % s.d);
-        if (i < s.n) m = fmax(m, fabs(s.x[i * s.ld + k] - mu[k]));
+        if (i < s.n) {
+            const double v = fabs(s.x[i * s.ld + k] - mu[k]);
+            bad |= !isfinite(v);
+            m = fmax(m, v);
+        }
     }
+    if (bad) atomicOr(&a.scal[si].nonfinite, 1ull);
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
     __syncthreads();

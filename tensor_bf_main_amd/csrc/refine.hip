@@ -6,38 +6,100 @@
 // compared with T, the largest double whose square root is <= eps, which is
 // the predicate sqrt(acc) <= eps of kernel_join.cu:194.  These are the only
 // FP64 steps of the low-precision paths besides the threshold computation.
+//
+// Layout of the work: one wavefront takes 64 band records, one per lane.  The
+// two rows of every record are streamed through LDS in chunks of RCH doubles:
+// the wavefront loads the 128 row segments of a chunk with coalesced 8-byte
+// loads (16 lanes per 128-byte segment) and stores them transposed,
+// lds[k][pair], so that each lane then runs its own sequential FMA chain with
+// conflict-free LDS reads.  One HBM/L2 line per row segment instead of one per
+// lane per word.
 #include "bfj_internal.h"
 #include <math.h>
 
-__global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
+#define RCH 16                 // doubles per row chunk (one 128-byte line)
+#define RPAD (BFJ_WAVE + 1)    // LDS row length: 65 doubles keeps both phases conflict-free
+#define RWAVES 2              // wavefronts per block: 2 x 17 KiB of staged rows
+
+__global__ __launch_bounds__(RWAVES * BFJ_WAVE) void k_refine(PassArgs a)
 {
+    __shared__ double sq[RWAVES][RCH][RPAD];
+    __shared__ double sc[RWAVES][RCH][RPAD];
+    __shared__ const double *sptr[RWAVES][2 * BFJ_WAVE];
+    __shared__ int sd[RWAVES][BFJ_WAVE];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t cnt = *a.band_count;
     const uint32_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < m; t += stride) {
-        const BandEntry be = a.band[t];
+    const uint32_t groups = (m + BFJ_WAVE - 1) / BFJ_WAVE;
+    const uint32_t wave_id = blockIdx.x * RWAVES + w, n_waves = gridDim.x * RWAVES;
+    for (uint32_t g = wave_id; g < groups; g += n_waves) {
+        const uint32_t t = g * BFJ_WAVE + lane;
+        const bool live = t < m;
+        BandEntry be;
+        be.i = be.j = 0;
+        be.z = 0.f;
+        be.meta = 0;
+        if (live) be = a.band[t];
         const int si = (int)(be.meta >> 16);
-        const uint32_t mask = be.meta & 0xffu;
         const SetDesc &s = a.sets[si];
-        const double *q = s.x + (int64_t)be.i * s.ld;
-        const double *c = s.x + (int64_t)be.j * s.ld;
+        const int d = live ? s.d : 0;
+        sptr[w][2 * lane] = s.x + (int64_t)be.i * s.ld;
+        sptr[w][2 * lane + 1] = s.x + (int64_t)be.j * s.ld;
+        sd[w][lane] = d;
+        int dmax = d;
+        for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o));
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         double acc = 0.0;
-        for (int k = 0; k < s.d; ++k) {
-            const double d = q[k] - c[k];
-            acc = fma(d, d, acc);
+        for (int k0 = 0; k0 < dmax; k0 += RCH) {
+            // Load: segment sg = (pair p, row r) covers k0..k0+RCH-1 of that row;
+            // 4 segments per instruction, 16 lanes each.
+#pragma unroll 4
+            for (int it = 0; it < 2 * BFJ_WAVE / 4; ++it) {
+                const int sg = it * 4 + (lane >> 4);
+                const int p = sg >> 1, r = sg & 1;
+                const int k = k0 + (lane & 15);
+                const double v = (k < sd[w][p]) ? sptr[w][sg][k] : 0.0;
+                if (r == 0) sq[w][lane & 15][p] = v;
+                else sc[w][lane & 15][p] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int kend = min(RCH, d - k0);
+            for (int kk = 0; kk < kend; ++kk) {
+                const double df = sq[w][kk][lane] - sc[w][kk][lane];
+                acc = fma(df, df, acc);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        for (int e = 0; e < s.n_eps; ++e)
-            if (((mask >> e) & 1u) && acc <= s.T[e]) atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + be.i], 1u);
-        const double sigma = a.scal[si].sigma;
-        const double approx = sigma * sigma * (a.hmax[s.row_off + be.i] - 2.0 * (double)be.z);
-        const double err = fabs(approx - acc);
-        if (err >= 0.0) atomicMax(&a.scal[si].err_bits, (unsigned long long)__double_as_longlong(err));
+        double err = -1.0;
+        if (live) {
+            const uint32_t mask = be.meta & 0xffu;
+            for (int e = 0; e < s.n_eps; ++e)
+                if (((mask >> e) & 1u) && acc <= s.T[e]) atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + be.i], 1u);
+            const double sigma = a.scal[si].sigma;
+            const double approx = sigma * sigma * (a.hmax[s.row_off + be.i] - 2.0 * (double)be.z);
+            err = fabs(approx - acc);
+        }
+        // Largest approximation error of the group: one atomic per dataset present.
+        const int si0 = __shfl(si, 0);
+        if (__ballot(live && si != si0) == 0ull) {
+            double mx = err;
+            for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+            if (lane == 0 && mx >= 0.0) atomicMax(&a.scal[si0].err_bits, (unsigned long long)__double_as_longlong(mx));
+        } else if (err >= 0.0) {
+            atomicMax(&a.scal[si].err_bits, (unsigned long long)__double_as_longlong(err));
+        }
     }
 }
 
 void bfj_launch_refine(const PassArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_refine, dim3(2048), dim3(BFJ_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_refine, dim3(4096), dim3(RWAVES * BFJ_WAVE), 0, st, a);
 }
 
 // Per-row counts (self excluded) into the caller's int64 array and the
