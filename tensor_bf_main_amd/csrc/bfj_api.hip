@@ -112,20 +112,31 @@ static int pick_kc(int dtype, int d)
     return 0;
 }
 
-static int lds_for(int dtype, int rows)
+int bfj_join_lds_f16(int, int, int);
+int bfj_join_lds_bf16(int, int, int);
+int bfj_join_lds_f8(int, int, int);
+int bfj_join_lds_f64(int, int, int);
+
+// Dynamic LDS of one join block: candidate tile ring + per-wavefront state.
+static int lds_for(int dtype, int kc, int rows, int n_eps)
 {
-    const int per_wave = BFJ_MAXEPS * rows * 4 + rows * 16 * bfj_acc_bytes(dtype);
-    return per_wave * (BFJ_BLOCK / BFJ_WAVE);
+    switch (dtype) {
+    case BFJ_FP16: return bfj_join_lds_f16(kc, rows, n_eps);
+    case BFJ_BF16: return bfj_join_lds_bf16(kc, rows, n_eps);
+    case BFJ_FP8: return bfj_join_lds_f8(kc, rows, n_eps);
+    case BFJ_FP64: return bfj_join_lds_f64(kc, rows, n_eps);
+    }
+    return 0;
 }
 
-// Default tile: the widest strip whose kernel still runs 2 waves per SIMD,
-// else the widest that was built.
-static int heuristic_rows(int dtype, int kc)
+// Default tile: the widest strip whose kernel still runs 2 blocks (8 waves) per
+// CU, else the widest that was built.
+static int heuristic_rows(int dtype, int kc, int n_eps)
 {
     int best = 0;
     for (int r : kRowOptions)
         if (bfj_join_has(dtype, kc, r)) {
-            const int occ = bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, r));
+            const int occ = bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps));
             if (occ >= 2 || best == 0) best = r;
         }
     return best;
@@ -289,18 +300,22 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
     for (int kc : kcs) {
         Group g;
         g.kc = kc;
+        int n_eps_max = 1;
+        for (auto &s : p.sets)
+            if (s.kc == kc) n_eps_max = std::max(n_eps_max, (int)s.n_eps);
         int rows = force_rows ? force_rows : ctx->tile_rows[dtype][kc];
-        if (!rows || !bfj_join_has(dtype, kc, rows)) rows = heuristic_rows(dtype, kc);
+        if (!rows || !bfj_join_has(dtype, kc, rows)) rows = heuristic_rows(dtype, kc, n_eps_max);
         if (!bfj_join_has(dtype, kc, rows)) return fail(BFJ_ERR_UNSUPPORTED, "no tile for dtype %d kc %d", dtype, kc);
         g.rows = rows;
-        g.lds = lds_for(dtype, rows);
+        g.lds = lds_for(dtype, kc, rows, n_eps_max);
         g.item0 = (int32_t)p.items.size();
         std::vector<std::pair<int64_t, WorkItem>> its;
         for (size_t si = 0; si < p.sets.size(); ++si) {
             const SetDesc &s = p.sets[si];
             if (s.kc != kc || s.row_end <= s.row_begin) continue;
             const int64_t rb_begin = s.row_begin / 16, rb_end = (s.row_end + 15) / 16;
-            for (int64_t rb = rb_begin; rb < rb_end; rb += rows / 16) {
+            // One item = one block strip: 4 wavefronts x R rows.
+            for (int64_t rb = rb_begin; rb < rb_end; rb += (BFJ_BLOCK / BFJ_WAVE) * rows / 16) {
                 WorkItem w;
                 w.set = (int32_t)si;
                 w.rb0 = (int32_t)rb;
@@ -428,9 +443,8 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
         L.rows = g.rows;
         L.lds_bytes = g.lds;
         const int occ = bfj_join_max_blocks_per_cu(p.dtype, g.kc, g.rows, g.lds);
-        const int waves_needed = g.n_items;
         const int persistent = std::max(1, occ) * ctx->n_cu;
-        L.grid_blocks = std::min(persistent, (waves_needed + 3) / 4);
+        L.grid_blocks = std::min(persistent, g.n_items);
         if (L.grid_blocks < 1) L.grid_blocks = 1;
         const int rc = bfj_join_launch(L, j, st);
         if (rc == 1) return fail(BFJ_ERR_INTERNAL, "join kernel dtype %d kc %d rows %d not built", p.dtype, g.kc, g.rows);

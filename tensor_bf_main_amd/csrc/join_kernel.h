@@ -6,26 +6,28 @@
 // samples they were meant to grow into (wmmaTensorCoresGemm.cu:97-296,
 // dmmaTensorCoresGemm.cu:113-287), with one gfx950 design:
 //
-//   * A wavefront owns a strip of R query rows.  Their operand fragments stay
-//     in VGPRs for the whole pass (R/16 row blocks x KC chunks).
-//   * It streams every candidate block of 16 rows: KC fragment loads, each one
-//     coalesced 1-2 KiB read of the pre-packed operand array.
+//   * A block of 4 wavefronts owns a strip of 4R query rows; each wavefront
+//     keeps the operand fragments of its R rows in VGPRs for the whole pass.
+//   * The block streams every candidate block of 16 rows through a
+//     double-buffered LDS tile: one coalesced load per block instead of one per
+//     wavefront (the reference reads candidates from global memory in every
+//     warp, kernel_join.cu:432), one barrier per candidate block.
 //   * The accumulator of each 16x16 tile starts at C0 = g_i - n_j, so after
 //     the MFMAs Z = 0.5*(H_i - d2_approx(i, j)) with no further arithmetic: the
 //     squared norms and the threshold are folded into the initial value
 //     (the reference pre-loads candidate chunk norms the same way,
 //     kernel_join.cu:433-437).
-//   * Z < 0 means "certainly out".  One max-reduction over the tile and one
-//     wave vote skip almost every tile.  Tiles with some Z >= 0 go to the slow
-//     path: certain-in pairs bump per-row counters in LDS, pairs inside the
-//     error band are appended to the FP64 refine list (one atomic per wave).
+//   * Z < 0 means "certainly out".  One max tree and one wave vote skip almost
+//     every tile.  Hot tiles are parked in LDS and tested exactly against the
+//     per-row thresholds, also held in LDS; certain-in pairs bump per-row LDS
+//     counters, band pairs are staged in LDS and flushed to the refine list 64
+//     at a time.  The hot path never waits on global memory.
 //   * Per-row counts are flushed with one global atomic per row per strip,
 //     instead of one atomic per hit (kernel_join.cu:484, :662).
 #pragma once
 #include "bfj_internal.h"
 
 struct TrF16 {
-    static constexpr bool prefetch = true; // double-buffer the candidate fragments
     static constexpr int dtype = BFJ_FP16;
     typedef bfj_h8 frag_t;
     typedef bfj_f4 acc_t;
@@ -39,7 +41,6 @@ struct TrF16 {
 };
 
 struct TrBF16 {
-    static constexpr bool prefetch = true; // double-buffer the candidate fragments
     static constexpr int dtype = BFJ_BF16;
     typedef bfj_b8 frag_t;
     typedef bfj_f4 acc_t;
@@ -52,7 +53,6 @@ struct TrBF16 {
 };
 
 struct TrF8 {
-    static constexpr bool prefetch = true; // double-buffer the candidate fragments
     static constexpr int dtype = BFJ_FP8;
     typedef bfj_i8 frag_t;
     typedef bfj_f4 acc_t;
@@ -66,7 +66,6 @@ struct TrF8 {
 };
 
 struct TrF64 {
-    static constexpr bool prefetch = false; // 64-cycle MFMAs hide the loads of one block
     static constexpr int dtype = BFJ_FP64;
     typedef bfj_d2 frag_t; // two K=4 steps of v_mfma_f64_16x16x4
     typedef bfj_d4 acc_t;
@@ -87,36 +86,94 @@ template <> struct Pair2<double> { typedef double2 T; };
 
 __device__ inline int bfj_lane() { return __lane_id(); }
 
+#define BFJ_JWAVES (BFJ_BLOCK / BFJ_WAVE) // wavefronts per join block
+#define BFJ_STAGE 128                     // band records staged per wavefront
+
+// LDS layout of one join block (bytes), shared by the kernel and the host.
 template <class Tr, int KC, int R>
-__device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, int rb0, uint32_t *lcnt,
-                                           typename Tr::val_t *zt)
+struct JoinLds {
+    typedef typename Tr::val_t val_t;
+    static constexpr int FB = (int)sizeof(typename Tr::frag_t);
+    static constexpr int TILE = KC * BFJ_WAVE * FB;              // candidate fragments of one 16-row block
+    static constexpr int BUF = TILE + 16 * (int)sizeof(val_t) + 16; // + norms + FP8 scales
+    static constexpr int RING = 2 * BUF;
+    static constexpr int RB = R / 16;
+    // per wavefront
+    static constexpr int ZT = RB * BFJ_WAVE * 4 * (int)sizeof(val_t);   // parked hot tiles
+    static constexpr int STG = BFJ_STAGE * (int)sizeof(BandEntry) + 16; // staged band records + count
+    __host__ __device__ static constexpr int cnt_bytes(int n_eps) { return n_eps * R * 4; }
+    __host__ __device__ static constexpr int thr_bytes(int n_eps) { return n_eps * R * 2 * (int)sizeof(val_t); }
+    __host__ __device__ static constexpr int per_wave(int n_eps)
+    {
+        return ZT + STG + ((cnt_bytes(n_eps) + thr_bytes(n_eps) + 15) / 16) * 16;
+    }
+    __host__ __device__ static constexpr int total(int n_eps) { return RING + BFJ_JWAVES * per_wave(n_eps); }
+};
+
+// Copy the staged band records of this wavefront to the global refine list.
+__device__ inline void flush_band(const JoinArgs &a, BandEntry *stg, uint32_t *stg_n, int lane)
+{
+    const uint32_t cnt = *stg_n;
+    if (cnt == 0) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(a.band_count, cnt);
+    base = __shfl(base, 0);
+    for (uint32_t t = lane; t < cnt; t += BFJ_WAVE)
+        if (base + t < a.band_capacity) a.band[base + t] = stg[t];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) *stg_n = 0;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <class Tr, int KC, int R>
+__device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, int rb0, unsigned char *lds)
 {
     typedef typename Tr::frag_t frag_t;
     typedef typename Tr::acc_t acc_t;
     typedef typename Tr::val_t val_t;
     typedef typename Pair2<val_t>::T pair_t;
+    typedef JoinLds<Tr, KC, R> L;
     constexpr int RB = R / 16;
     const int lane = bfj_lane();
+    const int wave = threadIdx.x / BFJ_WAVE;
+    const int n_eps = s.n_eps;
     const int64_t nblk = s.npad / 16;
+    const int64_t wrb0 = (int64_t)rb0 + wave * RB; // first row block of this wavefront
     const frag_t *P = reinterpret_cast<const frag_t *>(a.pack + s.pack_off);
     const val_t *nq = reinterpret_cast<const val_t *>(a.nq) + s.row_off;
     const val_t *gq = reinterpret_cast<const val_t *>(a.gq) + s.row_off;
     const pair_t *ov = reinterpret_cast<const pair_t *>(a.ov) + s.ov_off;
     const uint8_t *sc = (Tr::dtype == BFJ_FP8) ? a.scales + s.row_off : nullptr;
 
-    // Strip fragments and per-row initial values, resident for the whole pass.
+    unsigned char *mine = lds + L::RING + wave * L::per_wave(n_eps);
+    val_t *zt = reinterpret_cast<val_t *>(mine);
+    BandEntry *stg = reinterpret_cast<BandEntry *>(mine + L::ZT);
+    uint32_t *stg_n = reinterpret_cast<uint32_t *>(mine + L::ZT + BFJ_STAGE * sizeof(BandEntry));
+    uint32_t *lcnt = reinterpret_cast<uint32_t *>(mine + L::ZT + L::STG);
+    pair_t *thr = reinterpret_cast<pair_t *>(mine + L::ZT + L::STG + L::cnt_bytes(n_eps));
+
+    // Per-row state of this wavefront's rows: counters, thresholds (LDS), fragments
+    // and initial values (VGPRs).
+    for (int t = lane; t < n_eps * R; t += BFJ_WAVE) {
+        lcnt[t] = 0;
+        const int li = t % R, e = t / R;
+        const int64_t i = wrb0 * 16 + li;
+        thr[e * R + li] = (i < s.npad) ? ov[i * n_eps + e] : pair_t{(val_t)3e38, (val_t)3e38};
+    }
+    if (lane == 0) *stg_n = 0;
     frag_t A[RB][KC];
     val_t G[RB][4];
     int SA[RB];
+    bool any_rows = false;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
-        const int64_t blk = rb0 + rb;
+        const int64_t blk = wrb0 + rb;
         const bool ok = blk < nblk;
+        any_rows |= ok;
 #pragma unroll
-        for (int c = 0; c < KC; ++c) {
-            if (ok) A[rb][c] = P[(blk * KC + c) * BFJ_WAVE + lane];
-            else A[rb][c] = frag_t{};
-        }
+        for (int c = 0; c < KC; ++c) A[rb][c] = ok ? P[(blk * KC + c) * BFJ_WAVE + lane] : frag_t{};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int64_t row = blk * 16 + Tr::out_row(lane, r);
@@ -125,156 +182,157 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         SA[rb] = (Tr::dtype == BFJ_FP8 && ok) ? (int)sc[blk * 16 + (lane & 15)] : 127;
     }
 
-    const int n_eps = s.n_eps;
-    frag_t B[KC];
-    val_t nj = nq[lane & 15];
-    int SB = (Tr::dtype == BFJ_FP8) ? (int)sc[lane & 15] : 127;
-    if constexpr (Tr::prefetch) {
+    // Candidate tiles: 16-byte units spread over the block's threads.
+    constexpr int UNITS = L::TILE / 16;
+    constexpr int UPT = (UNITS + BFJ_BLOCK - 1) / BFJ_BLOCK;
+    const uint4 *src = reinterpret_cast<const uint4 *>(P);
+    uint4 pre[UPT];
+    val_t pre_n = (val_t)0;
+    uint8_t pre_s = 127;
+    auto fetch = [&](int64_t cb) {
 #pragma unroll
-        for (int c = 0; c < KC; ++c) B[c] = P[c * BFJ_WAVE + lane];
-    }
-
-    for (int64_t cb = 0; cb < nblk; ++cb) {
-        // Prefetch the next candidate block (fragments, norms, scales) while this
-        // one is in the MFMA pipe.
-        frag_t Bn[Tr::prefetch ? KC : 1];
-        const int64_t nb = (cb + 1 < nblk) ? cb + 1 : cb;
-        const val_t njn = nq[nb * 16 + (lane & 15)];
-        const int SBn = (Tr::dtype == BFJ_FP8) ? (int)sc[nb * 16 + (lane & 15)] : 127;
-        if constexpr (Tr::prefetch) {
-#pragma unroll
-            for (int c = 0; c < KC; ++c) Bn[c] = P[(nb * KC + c) * BFJ_WAVE + lane];
-        } else {
-#pragma unroll
-            for (int c = 0; c < KC; ++c) B[c] = P[(cb * KC + c) * BFJ_WAVE + lane];
+        for (int u = 0; u < UPT; ++u) {
+            const int unit = threadIdx.x + u * BFJ_BLOCK;
+            if (unit < UNITS) pre[u] = src[cb * UNITS + unit];
         }
+        if (threadIdx.x < 16) {
+            pre_n = nq[cb * 16 + threadIdx.x];
+            if (Tr::dtype == BFJ_FP8) pre_s = sc[cb * 16 + threadIdx.x];
+        }
+    };
+    auto stash = [&](int buf) {
+        unsigned char *b = lds + buf * L::BUF;
+#pragma unroll
+        for (int u = 0; u < UPT; ++u) {
+            const int unit = threadIdx.x + u * BFJ_BLOCK;
+            if (unit < UNITS) reinterpret_cast<uint4 *>(b)[unit] = pre[u];
+        }
+        if (threadIdx.x < 16) {
+            reinterpret_cast<val_t *>(b + L::TILE)[threadIdx.x] = pre_n;
+            b[L::TILE + 16 * sizeof(val_t) + threadIdx.x] = pre_s;
+        }
+    };
 
-        acc_t acc[RB];
+    fetch(0);
+    stash(0);
+    // Drain the strip's own loads here, so that inside the loop the only
+    // outstanding vector-memory operations are the next tile's prefetch and the
+    // MFMAs never wait on it.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int64_t cb = 0; cb < nblk; ++cb) {
+        const int buf = (int)(cb & 1);
+        if (cb + 1 < nblk) fetch(cb + 1); // in flight during this block's MFMAs
+        if (any_rows) {
+            const unsigned char *b = lds + buf * L::BUF;
+            frag_t B[KC];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) acc[rb][r] = G[rb][r] - nj;
-#pragma unroll
-        for (int c = 0; c < KC; ++c)
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][c], B[c], acc[rb], SA[rb], SB);
+            for (int c = 0; c < KC; ++c) B[c] = reinterpret_cast<const frag_t *>(b)[c * BFJ_WAVE + lane];
+            const val_t nj = reinterpret_cast<const val_t *>(b + L::TILE)[lane & 15];
+            const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::TILE + 16 * sizeof(val_t) + (lane & 15)] : 127;
 
-        // Inputs are NaN-free (the pass rejects non-finite rows), so with
-        // -fno-honor-nans this is a tree of v_max3.  -0.0 >= 0 holds, so a zero Z
-        // of either sign reaches the exact slow-path tests.
-        val_t m = acc[0][0];
+            acc_t acc[RB];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
+            for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) m = fmax(m, acc[rb][r]);
+                for (int r = 0; r < 4; ++r) acc[rb][r] = G[rb][r] - nj;
+#pragma unroll
+            for (int c = 0; c < KC; ++c)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][c], B[c], acc[rb], SA[rb], SB);
 
-        if (__builtin_expect(__ballot(m >= (val_t)0) != 0ull, 0)) {
-            // Slow path, taken by the few tiles where some pair is not certainly
-            // out: park only the hot 16x16 tiles in this wavefront's LDS (one
-            // 16-byte store per lane each) and walk them with a rolled loop, so
-            // the rare path costs neither registers nor code in the fast one.
-            uint32_t hot = 0;
+            // Inputs are NaN-free (the pass rejects non-finite rows), so with
+            // -fno-honor-nans this is a tree of v_max3.  -0.0 >= 0 holds, so a zero Z
+            // of either sign reaches the exact tests below.
+            val_t m = acc[0][0];
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                const val_t mr = fmax(fmax(acc[rb][0], acc[rb][1]), fmax(acc[rb][2], acc[rb][3]));
-                if (__ballot(mr >= (val_t)0) != 0ull) {
-                    hot |= 1u << rb;
-                    *reinterpret_cast<acc_t *>(zt + (rb * BFJ_WAVE + lane) * 4) = acc[rb];
-                }
-            }
-            const int64_t col = cb * 16 + (lane & 15);
-            const bool col_ok = col < s.n;
-            hot = (uint32_t)__builtin_amdgcn_readfirstlane((int)hot);
-#pragma unroll 1
-            for (int q = 0; q < RB * 4; ++q) {
-                if (!((hot >> (q >> 2)) & 1u)) {
-                    q |= 3; // skip the rest of this row block
-                    continue;
-                }
-                const val_t z = zt[((q >> 2) * BFJ_WAVE + lane) * 4 + (q & 3)];
-                const bool cand = z >= (val_t)0 && col_ok;
-                if (__ballot(cand) == 0ull) continue;
-                const int li = (q >> 2) * 16 + Tr::out_row(lane, q & 3);
-                const int64_t i = (int64_t)rb0 * 16 + li;
-                uint32_t band = 0;
-                if (cand) {
-                    for (int e = 0; e < n_eps; ++e) {
-                        const pair_t t = ov[i * n_eps + e];
-                        if (z >= t.y) atomicAdd(&lcnt[e * R + li], 1u);
-                        else if (z >= t.x) band |= 1u << e;
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) m = fmax(m, acc[rb][r]);
+
+            if (__builtin_expect(__ballot(m >= (val_t)0) != 0ull, 0)) {
+                // Hot tiles: park them (one 16-byte LDS store per lane each) and walk
+                // them with a rolled loop, so this rare path costs no registers.
+                uint32_t hot = 0;
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    const val_t mr = fmax(fmax(acc[rb][0], acc[rb][1]), fmax(acc[rb][2], acc[rb][3]));
+                    if (__ballot(mr >= (val_t)0) != 0ull) {
+                        hot |= 1u << rb;
+                        *reinterpret_cast<acc_t *>(zt + (rb * BFJ_WAVE + lane) * 4) = acc[rb];
                     }
                 }
-                const unsigned long long bal = __ballot(band != 0u);
-                if (bal) {
-                    const int leader = __ffsll((long long)bal) - 1;
-                    uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(a.band_count, (uint32_t)__popcll(bal));
-                    base = __shfl(base, leader);
-                    if (band) {
-                        const uint32_t idx = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-                        if (idx < a.band_capacity) {
+                hot = (uint32_t)__builtin_amdgcn_readfirstlane((int)hot);
+                const int64_t col = cb * 16 + (lane & 15);
+                const bool col_ok = col < s.n;
+#pragma unroll 1
+                for (int q = 0; q < RB * 4; ++q) {
+                    if (!((hot >> (q >> 2)) & 1u)) {
+                        q |= 3; // rest of this row block is cold
+                        continue;
+                    }
+                    const val_t z = zt[((q >> 2) * BFJ_WAVE + lane) * 4 + (q & 3)];
+                    const bool cand = z >= (val_t)0 && col_ok;
+                    if (__ballot(cand) == 0ull) continue;
+                    if (*stg_n > BFJ_STAGE - BFJ_WAVE) flush_band(a, stg, stg_n, lane);
+                    const int li = (q >> 2) * 16 + Tr::out_row(lane, q & 3);
+                    uint32_t band = 0;
+                    if (cand) {
+                        for (int e = 0; e < n_eps; ++e) {
+                            const pair_t t = thr[e * R + li];
+                            if (z >= t.y) atomicAdd(&lcnt[e * R + li], 1u);
+                            else if (z >= t.x) band |= 1u << e;
+                        }
+                        if (band) {
+                            const uint32_t idx = atomicAdd(stg_n, 1u);
                             BandEntry be;
-                            be.i = (uint32_t)i;
+                            be.i = (uint32_t)(wrb0 * 16 + li);
                             be.j = (uint32_t)col;
                             be.z = (float)z;
                             be.meta = band | ((uint32_t)s.set_id << 16);
-                            a.band[idx] = be;
+                            stg[idx] = be;
                         }
                     }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
         }
-        nj = njn;
-        SB = SBn;
-        if constexpr (Tr::prefetch) {
-#pragma unroll
-            for (int c = 0; c < KC; ++c) B[c] = Bn[c];
-        }
+        if (cb + 1 < nblk) stash(buf ^ 1);
+        __syncthreads();
     }
 
-    // Flush per-row counts of the strip: one global atomic per non-zero row.
-    __threadfence_block();
+    // Flush this wavefront's per-row counts and staged band records.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int t = lane; t < n_eps * R; t += BFJ_WAVE) {
         const uint32_t v = lcnt[t];
         if (v) {
             const int e = t / R, li = t % R;
-            const int64_t i = (int64_t)rb0 * 16 + li;
-            atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + i], v);
-            lcnt[t] = 0;
+            atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + wrb0 * 16 + li], v);
         }
     }
-    __threadfence_block();
+    flush_band(a, stg, stg_n, lane);
 }
 
-// Dynamic LDS per wavefront: BFJ_MAXEPS x R row counters, then the R x 16 tile
-// of Z values parked by the slow path.
-template <class Tr, int R>
-__host__ __device__ constexpr int join_lds_per_wave()
-{
-    return BFJ_MAXEPS * R * 4 + R * 16 * (int)sizeof(typename Tr::val_t);
-}
-
-// Persistent kernel: every wavefront takes strips from a global queue until it
-// is empty; the grid may also be sized to one wavefront per strip.
+// Persistent kernel: every block takes 4R-row strips from a global queue until
+// it is empty.  All waves of a block stay together (the candidate tiles are
+// shared), so the queue is read by one thread and broadcast through LDS.
 template <class Tr, int KC, int R>
 __global__ __launch_bounds__(BFJ_BLOCK) void k_join(JoinArgs a)
 {
-    extern __shared__ unsigned char lds_raw[];
-    const int wave = threadIdx.x / BFJ_WAVE;
-    const int lane = bfj_lane();
-    unsigned char *mine = lds_raw + wave * join_lds_per_wave<Tr, R>();
-    uint32_t *lcnt = reinterpret_cast<uint32_t *>(mine);
-    typename Tr::val_t *zt = reinterpret_cast<typename Tr::val_t *>(mine + BFJ_MAXEPS * R * 4);
-    for (int t = lane; t < BFJ_MAXEPS * R; t += BFJ_WAVE) lcnt[t] = 0;
-    __threadfence_block();
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    __shared__ uint32_t item_s;
     for (;;) {
-        uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(a.item_counter, 1u);
-        // Wave-uniform from here on: descriptor loads become scalar loads.
-        item = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(item, 0));
+        if (threadIdx.x == 0) item_s = atomicAdd(a.item_counter, 1u);
+        __syncthreads();
+        const uint32_t item = (uint32_t)__builtin_amdgcn_readfirstlane((int)item_s);
+        __syncthreads();
         if (item >= (uint32_t)a.n_items) break;
         const WorkItem w = a.items[item];
         const SetDesc s = a.sets[w.set];
-        join_strip<Tr, KC, R>(a, s, w.rb0, lcnt, zt);
+        join_strip<Tr, KC, R>(a, s, w.rb0, lds_raw);
     }
 }

@@ -204,3 +204,25 @@ def test_workload3_full_fp16_vs_fp64(ctx):
     rows = np.random.default_rng(4242).choice(n, 1000, replace=False)
     oc = oracle.join_rows(x, rows, eps)[0]
     np.testing.assert_array_equal(a16[rows], oc)
+
+
+@pytest.mark.parametrize("mode,dtype", [(1, "fp64"), (2, "fp64"), (3, "fp16"), (4, "fp16"), (7, "fp64"),
+                                        (8, "fp16"), (9, "bf16"), (10, "fp8")])
+def test_cli_drop_in(tmp_path, mode, dtype):
+    """bfjoin_main keeps the reference CLI (main.cu:16-128): same arguments, console
+    lines and CSV log; every searchMode returns the exact total."""
+    import subprocess
+
+    gold = json.load(open(os.path.join(GOLDEN, "expected_counts.json")))["test128.txt"]
+    k = gold["eps"].index(0.05)
+    log = tmp_path / "log.csv"
+    counts = tmp_path / "counts.txt"
+    r = subprocess.run([bfj.CLI_PATH, os.path.join(GOLDEN, "test128.txt"), "0.05", str(mode), "0", "--dim", "18",
+                        "--log", str(log), "--counts", str(counts)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert f"[Main | Result] ~ Total result set size: {gold['totals'][k]}" in r.stdout
+    rows = log.read_text().splitlines()
+    assert rows[0].startswith("Dataset, epsilon, searchMode, executionTime, totalNeighbors, inputDim, computeDim")
+    f = [v.strip() for v in rows[1].split(",")]
+    assert int(f[2]) == mode and int(f[4]) == gold["totals"][k] and f[11] == dtype
+    assert [int(v) for v in counts.read_text().split()] == gold["counts"][k]
