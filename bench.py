@@ -1,15 +1,22 @@
 """Benchmark: brute-force Euclidean epsilon self-join on one MI355X.
 
-Headline workload: n = 1,000,000 points, d = 128, Gaussian mixture (64 centres
-uniform in [0,1)^128, sigma 0.03 per dimension), eps set for a mean of ~100
-neighbours.  One step = one full pass of the join (centre/cast/norms/thresholds,
-MFMA join with fused epsilon epilogue, FP64 band refine, count finalisation) over
-the dataset already resident in HBM.  The FP16-MFMA path is the measured value;
-the plain FP64-MFMA join runs on the same data for the speedup, and the two must
-give identical per-point counts (plus 1,000 seeded rows against the CPU oracle).
+Headline (default, --workload 3): n = 1,000,000 points, d = 128, Gaussian mixture
+(64 centres uniform in [0,1)^128, sigma 0.03 per dimension), eps set for a mean of
+~100 neighbours.  One step = one full pass of the join (centre/cast/norms/
+thresholds, MFMA join with fused epsilon epilogue, FP64 band refine, count
+finalisation) over the dataset already resident in HBM.  The FP16-MFMA path is
+the measured value; the plain FP64-MFMA join runs on the same data for the
+speedup, and the two must give identical per-point counts (plus seeded rows
+against the CPU oracle).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--skip-fp64] [--skip-cpu]
-Prints one JSON line.
+Other workloads (one JSON line each, same keys):
+  --workload 2   n=100,000 d=128 uniform, FP64-MFMA join (exact vs oracle rows)
+  --workload 4   n=500,000 d=256 unit-norm rows with 5% near-duplicates, --dtype bf16|fp8|fp16,
+                 compared with the FP64-MFMA join
+  --workload 5   512 datasets (n log-uniform in [1000, 65536], d in {16,64,128,200}),
+                 4 eps each, FP16: one persistent hipGraph launch vs per-dataset launches
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3] [--n N] ...
 """
 import argparse
 import json
@@ -29,13 +36,159 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def roofline(flops, join_ms, dtype):
+    achieved = flops / (join_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
+            "frac": achieved / PEAK_TFLOPS[dtype], "traffic": None,
+            "kernel": "k_join (MFMA join + fused epsilon epilogue)"}
+
+
+def cpu_baseline(x, eps, rows_n, got_counts, seed=4242):
+    """Oracle on a bounded sample of query rows; returns (baseline dict, mismatches)."""
+    from oracle import oracle
+
+    n = x.shape[0]
+    rows = np.random.default_rng(seed).choice(n, rows_n, replace=False)
+    log(f"cpu oracle on {rows_n} rows x {n} candidates, {oracle.num_threads()} threads")
+    t0 = time.perf_counter()
+    oc = oracle.join_rows(x, rows, eps)
+    tc = time.perf_counter() - t0
+    mism = int((got_counts[:, rows] != oc).sum())
+    return {"value": rows_n * float(n) / tc, "unit": "pair-distances/s", "cores": oracle.num_threads(),
+            "kind": "port", "sample": f"{rows_n} seeded query rows x all {n} candidates, FP64 direct form "
+                                      f"(OpenMP), {tc:.1f} s"}, mism
+
+
+def single_dataset(args, ctx, x, eps, dtype, metric, workload):
+    n, d = x.shape
+    e = np.atleast_1d(eps)
+    xd = ctx.to_device(x)  # rows resident in HBM before the timed region
+    c_main = ctx.empty((e.size, n), np.int64)
+
+    def step(dt, counts):
+        # Each call enqueues the whole pass on the library stream and returns after
+        # that stream has synchronised, so the wall clock brackets complete steps.
+        return ctx.self_join_device(xd, e, dtype=dt, counts=counts)
+
+    log(f"warmup {args.warmup} x {dtype}")
+    for _ in range(args.warmup):
+        step(dtype, c_main)
+    join_ms, last = [], None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        last = step(dtype, c_main)
+        join_ms.append(last.stats["ms_join"])
+    t_main = (time.perf_counter() - t0) / args.steps
+    pairs = float(n) * float(n)
+    log(f"{dtype}: {t_main * 1e3:.1f} ms/step, join kernel {np.mean(join_ms):.1f} ms, "
+        f"refine {last.stats['ms_refine']:.1f} ms, band {last.stats['band_pairs']}")
+    out = {
+        "metric": metric, "value": pairs / t_main, "unit": "pair-distances/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_main * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+        "config": {"workload": workload, "n": n, "d": d, "eps": [float(v) for v in e],
+                   "mean_neighbours": float(last.totals[0] - n) / n, "path": f"{dtype}-mfma+fp64-refine"},
+        "total_pairs_within_eps": [int(t) for t in last.totals],
+        "band_pairs": int(last.stats["band_pairs"]),
+        "band_fraction": last.stats["band_pairs"] / pairs,
+        "max_abs_d2_err": last.stats["max_abs_d2_err"],
+        "max_band_halfwidth": last.stats["max_band_halfwidth"],
+        "stage_ms": {k: last.stats[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
+        "tile_rows": last.stats["tile_rows"],
+    }
+    flops = 2.0 * d * pairs  # algorithmic: true d, FMA = 2
+    out["roofline"] = roofline(flops, float(np.mean(join_ms)), dtype)
+    got = c_main.numpy()
+    mismatches = None
+    if dtype != "fp64" and not args.skip_fp64:
+        c64 = ctx.empty((e.size, n), np.int64)
+        log("fp64 warmup")
+        step("fp64", c64)
+        t0 = time.perf_counter()
+        r64 = None
+        for _ in range(args.fp64_steps):
+            r64 = step("fp64", c64)
+        t64 = (time.perf_counter() - t0) / args.fp64_steps
+        mismatches = int((got != c64.numpy()).sum()) + int((r64.totals != last.totals).sum())
+        out["fp64_ms_per_step"] = t64 * 1e3
+        out["fp64_pairs_per_s"] = pairs / t64
+        out["speedup_vs_fp64"] = t64 / t_main
+        out["fp64_join_tflops"] = flops / (r64.stats["ms_join"] * 1e-3) / 1e12
+        log(f"fp64: {t64 * 1e3:.1f} ms/step; speedup {t64 / t_main:.2f}x; mismatches {mismatches}")
+    if not args.skip_cpu:
+        rows_n = args.cpu_rows or max(64, min(12000, int(6.0e9 / n * 128 / d)))
+        out["cpu_baseline"], sm = cpu_baseline(x, e, rows_n, got)
+        out["oracle_sample_mismatches"] = sm
+        mismatches = (mismatches or 0) + sm
+    out["mismatches"] = mismatches
+    return out
+
+
+def batch_workload(args, ctx):
+    from tensor_bf_main_amd import workloads as wl
+
+    log(f"generating workload 5: {args.sets} datasets")
+    sets = wl.batch(n_sets=args.sets)
+    dev = [{"data": ctx.to_device(s["data"]), "eps": s["eps"],
+            "counts": ctx.empty((len(s["eps"]), s["data"].shape[0]), np.int64)} for s in sets]
+    pairs = float(sum(s["data"].shape[0] ** 2 for s in sets))
+    flops = float(sum(2.0 * s["data"].shape[1] * s["data"].shape[0] ** 2 for s in sets))
+    log("plan: autotune + capture")
+    t0 = time.perf_counter()
+    plan = ctx.plan(dev, dtype=args.dtype)
+    t_plan = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        plan.run()
+    st, t0 = None, time.perf_counter()
+    for _ in range(args.steps):
+        st = plan.run()
+    t_graph = (time.perf_counter() - t0) / args.steps
+    graph_counts = [dv["counts"].numpy() for dv in dev]
+    log(f"graph: {t_graph * 1e3:.1f} ms/step ({pairs / t_graph:.3e} pairs/s)")
+    # The same work as one call per dataset (eager, per-dataset launches + sync).
+    for dv in dev[:8]:
+        ctx.self_join_device(dv["data"], dv["eps"], dtype=args.dtype, counts=dv["counts"])
+    t0 = time.perf_counter()
+    for dv in dev:
+        ctx.self_join_device(dv["data"], dv["eps"], dtype=args.dtype, counts=dv["counts"])
+    t_each = time.perf_counter() - t0
+    mismatches = sum(int((dv["counts"].numpy() != g).sum()) for dv, g in zip(dev, graph_counts))
+    log(f"per-dataset launches: {t_each * 1e3:.1f} ms; graph speedup {t_each / t_graph:.2f}x")
+    out = {"metric": f"pair-distances/s, {len(sets)} datasets x 4 eps in one persistent hipGraph launch",
+           "value": pairs / t_graph, "unit": "pair-distances/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": t_graph * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": args.dtype, "data": "synthetic",
+           "config": {"workload": f"batch_{len(sets)}", "sets": len(sets), "total_pairs": pairs,
+                      "dims": sorted({int(s["data"].shape[1]) for s in sets})},
+           "per_dataset_launches_ms": t_each * 1e3, "per_dataset_pairs_per_s": pairs / t_each,
+           "graph_speedup_vs_per_dataset": t_each / t_graph, "plan_create_s": t_plan,
+           "band_pairs": int(st["band_pairs"]),
+           "stage_ms": {k: st[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
+           "roofline": roofline(flops, st["ms_join"], args.dtype), "mismatches_graph_vs_eager": mismatches}
+    if not args.skip_cpu:
+        from oracle import oracle
+
+        # Exact counts of a few whole datasets against the oracle.
+        pick = np.random.default_rng(7).choice(len(sets), min(6, len(sets)), replace=False)
+        mism = 0
+        for k in pick:
+            oc, _ = oracle.join(sets[k]["data"], sets[k]["eps"])
+            mism += int((graph_counts[k] != oc).sum())
+        out["oracle_sets_checked"] = len(pick)
+        out["oracle_mismatches"] = mism
+    plan.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--dtype", default="fp16", choices=["fp16", "bf16", "fp8", "fp64"])
+    ap.add_argument("--workload", type=int, default=3, choices=[2, 3, 4, 5])
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--sets", type=int, default=512)
+    ap.add_argument("--dtype", default="", choices=["", "fp16", "bf16", "fp8", "fp64"])
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -49,95 +202,22 @@ def main():
     import tensor_bf_main_amd as bfj
     from tensor_bf_main_amd import workloads as wl
 
-    n, d = args.n, 128
-    log(f"generating workload 3: n={n} d={d}")
-    x, eps = wl.config(3, n=n)
     ctx = bfj.Context(0)
-    xd = ctx.to_device(x)  # rows resident in HBM before the timed region
-    c_main = ctx.empty((1, n), np.int64)
-
-    def step(dtype, counts):
-        # Each call enqueues the whole pass on the library stream and returns after
-        # that stream has synchronised, so the wall clock brackets complete steps.
-        return ctx.self_join_device(xd, eps, dtype=dtype, counts=counts)
-
-    log(f"warmup {args.warmup} x {args.dtype}")
-    for _ in range(args.warmup):
-        step(args.dtype, c_main)
-    join_ms, last = [], None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        last = step(args.dtype, c_main)
-        join_ms.append(last.stats["ms_join"])
-    t_main = (time.perf_counter() - t0) / args.steps
-    pairs = float(n) * float(n)
-    log(f"{args.dtype}: {t_main * 1e3:.1f} ms/step, join kernel {np.mean(join_ms):.1f} ms, "
-        f"band {last.stats['band_pairs']}")
-
-    out = {
-        "metric": "pair-distances/s at n=1M,d=128 (FP16-MFMA + FP64 band refine, exact counts)",
-        "value": pairs / t_main,
-        "unit": "pair-distances/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t_main * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": args.dtype,
-        "data": "synthetic",
-        "config": {"workload": "gmm_64c_sigma0.03", "n": n, "d": d, "eps": eps,
-                   "mean_neighbours": float(last.totals[0] - n) / n, "path": f"{args.dtype}-mfma+fp64-refine"},
-        "total_pairs_within_eps": int(last.totals[0]),
-        "band_pairs": int(last.stats["band_pairs"]),
-        "band_fraction": last.stats["band_pairs"] / pairs,
-        "max_abs_d2_err": last.stats["max_abs_d2_err"],
-        "max_band_halfwidth": last.stats["max_band_halfwidth"],
-        "stage_ms": {k: last.stats[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
-        "tile_rows": last.stats["tile_rows"],
-    }
-    flops = 2.0 * d * pairs  # algorithmic: true d, FMA = 2
-    achieved = flops / (np.mean(join_ms) * 1e-3) / 1e12
-    out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[args.dtype], "unit": "TFLOP/s",
-                       "frac": achieved / PEAK_TFLOPS[args.dtype], "traffic": None,
-                       "kernel": "k_join (MFMA join + fused epsilon epilogue)"}
-
-    mismatches = None
-    if not args.skip_fp64:
-        c64 = ctx.empty((1, n), np.int64)
-        log("fp64 warmup")
-        step("fp64", c64)
-        t0 = time.perf_counter()
-        r64 = None
-        for _ in range(args.fp64_steps):
-            r64 = step("fp64", c64)
-        t64 = (time.perf_counter() - t0) / args.fp64_steps
-        mismatches = int((c_main.numpy() != c64.numpy()).sum()) + int(r64.totals[0] != last.totals[0])
-        out["fp64_ms_per_step"] = t64 * 1e3
-        out["fp64_pairs_per_s"] = pairs / t64
-        out["speedup_vs_fp64"] = t64 / t_main
-        out["fp64_join_tflops"] = flops / (r64.stats["ms_join"] * 1e-3) / 1e12
-        log(f"fp64: {t64 * 1e3:.1f} ms/step; speedup {t64 / t_main:.2f}x; mismatches {mismatches}")
-
-    if not args.skip_cpu:
-        from oracle import oracle
-
-        rows_n = args.cpu_rows or max(64, min(12000, int(6.0e9 / n)))
-        rows = np.random.default_rng(4242).choice(n, rows_n, replace=False)
-        log(f"cpu oracle on {rows_n} rows x {n} candidates, {oracle.num_threads()} threads")
-        t0 = time.perf_counter()
-        oc = oracle.join_rows(x, rows, eps)
-        tc = time.perf_counter() - t0
-        got = c_main.numpy()[0][rows]
-        sample_mismatch = int((got != oc[0]).sum())
-        mismatches = (mismatches or 0) + sample_mismatch
-        out["cpu_baseline"] = {"value": rows_n * float(n) / tc, "unit": "pair-distances/s",
-                               "cores": oracle.num_threads(), "kind": "port",
-                               "sample": f"{rows_n} seeded query rows x all {n} candidates, FP64 direct form "
-                                         f"(OpenMP), {tc:.1f} s"}
-        out["oracle_sample_mismatches"] = sample_mismatch
-    out["mismatches"] = mismatches
+    if args.workload == 5:
+        args.dtype = args.dtype or "fp16"
+        out = batch_workload(args, ctx)
+    else:
+        n = args.n or {2: 100_000, 3: 1_000_000, 4: 500_000}[args.workload]
+        dtype = args.dtype or {2: "fp64", 3: "fp16", 4: "bf16"}[args.workload]
+        log(f"generating workload {args.workload}: n={n}")
+        x, eps = wl.config(args.workload, n=n)
+        names = {2: ("pair-distances/s at n=100k,d=128 (FP64-MFMA + FP64 band refine, exact counts)", "uniform"),
+                 3: ("pair-distances/s at n=1M,d=128 (FP16-MFMA + FP64 band refine, exact counts)",
+                     "gmm_64c_sigma0.03"),
+                 4: (f"pair-distances/s at n=500k,d=256 ({dtype.upper()}-MFMA + FP64 band refine, exact counts)",
+                     "unit_norm_5pct_dups")}
+        metric, wname = names[args.workload]
+        out = single_dataset(args, ctx, x, eps, dtype, metric, wname)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -782,6 +782,21 @@ int bfj_plan_create(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32
     std::vector<HostSet> hs;
     std::vector<size_t> xoff, ooff;
     RC(batch_hostsets(ctx, sets, n_sets, 1, hs, xoff, ooff));
+    // Tile shapes: every K-chunk group that has no tuned tile yet is autotuned on a
+    // synthetic problem of its largest dimension (capped at 32768 rows).
+    if (dtype >= BFJ_FP64 && dtype <= BFJ_FP8)
+        for (const HostSet &h : hs) {
+            const int kc = (h.d >= 1 && h.d <= BFJ_MAX_DIM) ? pick_kc(dtype, h.d) : 0;
+            if (!kc || ctx->tile_rows[dtype][kc]) continue;
+            int dmax = 1;
+            int64_t nmax = 256;
+            for (const HostSet &o : hs)
+                if (o.d >= 1 && o.d <= BFJ_MAX_DIM && pick_kc(dtype, o.d) == kc) {
+                    dmax = std::max(dmax, o.d);
+                    nmax = std::max(nmax, o.n);
+                }
+            RC(bfj_autotune(ctx, dtype, dmax, std::min<int64_t>(nmax, 32768), nullptr));
+        }
     bfj_plan *pl = new bfj_plan();
     pl->ctx = ctx;
     pl->sets.assign(sets, sets + n_sets);

@@ -59,6 +59,7 @@ struct SetScalars {
     unsigned long long err_bits;    // max |d2_approx - d2_fp64| over refined pairs (input space)
     unsigned long long halfband_bits; // max (H - L) / 2 (input space)
     unsigned long long nonfinite;     // non-zero if some row holds NaN or inf
+    unsigned long long all_mask;      // eps whose T exceeds every possible distance: all pairs hit
     unsigned long long total[BFJ_MAXEPS]; // totals, self included
 };
 

@@ -94,9 +94,18 @@ template <class Tr, int KC, int R>
 struct JoinLds {
     typedef typename Tr::val_t val_t;
     static constexpr int FB = (int)sizeof(typename Tr::frag_t);
-    static constexpr int TILE = KC * BFJ_WAVE * FB;              // candidate fragments of one 16-row block
-    static constexpr int BUF = TILE + 16 * (int)sizeof(val_t) + 16; // + norms + FP8 scales
-    static constexpr int RING = 2 * BUF;
+    static constexpr int TILE = KC * BFJ_WAVE * FB; // candidate fragments of one 16-row block
+    static constexpr int UNITS = TILE / 16;         // 16-byte units of a tile
+    // Direct-to-LDS loads per wavefront per tile: the fragment area is padded to a
+    // whole number of block-wide 16-byte loads so every wavefront issues the same
+    // count, which keeps the vmcnt bookkeeping a compile-time constant.
+    static constexpr int UPW = (UNITS + BFJ_BLOCK - 1) / BFJ_BLOCK;
+    static constexpr int FRAG = UPW * BFJ_BLOCK * 16;
+    static constexpr int NQB = 16 * (int)sizeof(val_t); // candidate norms of the tile
+    static constexpr int BUF = FRAG + NQB + 16;          // + FP8 scales
+    static constexpr int NB = BUF <= 4608 ? 4 : (BUF <= 9216 ? 3 : 2); // ring depth
+    static constexpr int LPT = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0); // DMA instructions per tile
+    static constexpr int RING = NB * BUF;
     static constexpr int RB = R / 16;
     // per wavefront
     static constexpr int ZT = RB * BFJ_WAVE * 4 * (int)sizeof(val_t);   // parked hot tiles
@@ -120,6 +129,8 @@ __device__ inline void flush_band(const JoinArgs &a, BandEntry *stg, uint32_t *s
     base = __shfl(base, 0);
     for (uint32_t t = lane; t < cnt; t += BFJ_WAVE)
         if (base + t < a.band_capacity) a.band[base + t] = stg[t];
+    // Drain: the tile pipeline counts only its own loads in vmcnt.
+    __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) *stg_n = 0;
@@ -182,54 +193,61 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         SA[rb] = (Tr::dtype == BFJ_FP8 && ok) ? (int)sc[blk * 16 + (lane & 15)] : 127;
     }
 
-    // Candidate tiles: 16-byte units spread over the block's threads.
-    constexpr int UNITS = L::TILE / 16;
-    constexpr int UPT = (UNITS + BFJ_BLOCK - 1) / BFJ_BLOCK;
-    const uint4 *src = reinterpret_cast<const uint4 *>(P);
-    uint4 pre[UPT];
-    val_t pre_n = (val_t)0;
-    uint8_t pre_s = 127;
-    auto fetch = [&](int64_t cb) {
+    // Candidate tiles stream through an NB-deep LDS ring filled by direct-to-LDS
+    // loads (global_load_lds, no VGPRs), NB-1 tiles ahead of the MFMAs.  Wavefront
+    // w loads units w*64 + 256u + lane of each tile (units past the tile re-read
+    // unit 0 into the padding), a quarter of the norms, and (FP8) a quarter of the
+    // scales: LPT instructions per tile for every wavefront.
+    typedef __attribute__((address_space(3))) void lds_void;
+    const unsigned char *srcb = reinterpret_cast<const unsigned char *>(P);
+    const unsigned char *nqb = reinterpret_cast<const unsigned char *>(nq);
+    auto issue = [&](int64_t t, int bi) {
+        unsigned char *b = lds + bi * L::BUF;
 #pragma unroll
-        for (int u = 0; u < UPT; ++u) {
-            const int unit = threadIdx.x + u * BFJ_BLOCK;
-            if (unit < UNITS) pre[u] = src[cb * UNITS + unit];
+        for (int u = 0; u < L::UPW; ++u) {
+            const int unit = u * BFJ_BLOCK + wave * BFJ_WAVE + lane;
+            const unsigned char *g = srcb + (t * L::UNITS + (unit < L::UNITS ? unit : 0)) * 16;
+            __builtin_amdgcn_global_load_lds(g, (lds_void *)(b + (u * BFJ_BLOCK + wave * BFJ_WAVE) * 16), 16, 0, 0);
         }
-        if (threadIdx.x < 16) {
-            pre_n = nq[cb * 16 + threadIdx.x];
-            if (Tr::dtype == BFJ_FP8) pre_s = sc[cb * 16 + threadIdx.x];
-        }
+        constexpr int NQW = L::NQB / BFJ_JWAVES; // norm bytes per wavefront
+        if (lane < NQW / 4)
+            __builtin_amdgcn_global_load_lds(nqb + t * L::NQB + wave * NQW + lane * 4,
+                                             (lds_void *)(b + L::FRAG + wave * NQW), 4, 0, 0);
+        if (Tr::dtype == BFJ_FP8 && lane == 0)
+            __builtin_amdgcn_global_load_lds(sc + t * 16 + wave * 4, (lds_void *)(b + L::FRAG + L::NQB + wave * 4), 4,
+                                             0, 0);
     };
-    auto stash = [&](int buf) {
-        unsigned char *b = lds + buf * L::BUF;
-#pragma unroll
-        for (int u = 0; u < UPT; ++u) {
-            const int unit = threadIdx.x + u * BFJ_BLOCK;
-            if (unit < UNITS) reinterpret_cast<uint4 *>(b)[unit] = pre[u];
-        }
-        if (threadIdx.x < 16) {
-            reinterpret_cast<val_t *>(b + L::TILE)[threadIdx.x] = pre_n;
-            b[L::TILE + 16 * sizeof(val_t) + threadIdx.x] = pre_s;
-        }
-    };
+    // vmcnt <= (NB-2)*LPT: of this wavefront's loads only the NB-2 newest tiles
+    // may still be in flight, so the oldest one (the tile about to be read) landed.
+    constexpr int WAIT_N = (L::NB - 2) * L::LPT;
+    constexpr int WAIT_IMM = (WAIT_N & 15) | ((WAIT_N >> 4) << 14) | (7 << 4) | (15 << 8);
+    static_assert(WAIT_N < 64, "vmcnt field");
 
-    fetch(0);
-    stash(0);
     // Drain the strip's own loads here, so that inside the loop the only
-    // outstanding vector-memory operations are the next tile's prefetch and the
-    // MFMAs never wait on it.
+    // outstanding vector-memory operations are the ring's loads.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+#pragma unroll
+    for (int t = 0; t < L::NB - 1; ++t) issue(t < nblk ? t : nblk - 1, t);
     for (int64_t cb = 0; cb < nblk; ++cb) {
-        const int buf = (int)(cb & 1);
-        if (cb + 1 < nblk) fetch(cb + 1); // in flight during this block's MFMAs
+        __builtin_amdgcn_s_waitcnt(WAIT_IMM);
+        // All quarters of tile cb landed (each wavefront waited for its own loads
+        // above); every wavefront is done with tile cb-1, whose fragments fed its
+        // MFMAs before it got here.  No release fence: the ring is written only by
+        // the direct-to-LDS loads, and a release would drain the ones in flight.
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        {
+            const int64_t t = cb + L::NB - 1; // re-uses the buffer of tile cb-1
+            issue(t < nblk ? t : nblk - 1, (int)(t % L::NB));
+        }
         if (any_rows) {
-            const unsigned char *b = lds + buf * L::BUF;
+            const unsigned char *b = lds + (int)(cb % L::NB) * L::BUF;
             frag_t B[KC];
 #pragma unroll
             for (int c = 0; c < KC; ++c) B[c] = reinterpret_cast<const frag_t *>(b)[c * BFJ_WAVE + lane];
-            const val_t nj = reinterpret_cast<const val_t *>(b + L::TILE)[lane & 15];
-            const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::TILE + 16 * sizeof(val_t) + (lane & 15)] : 127;
+            const val_t nj = reinterpret_cast<const val_t *>(b + L::FRAG)[lane & 15];
+            const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + (lane & 15)] : 127;
 
             acc_t acc[RB];
 #pragma unroll
@@ -299,9 +317,10 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
                 }
             }
         }
-        if (cb + 1 < nblk) stash(buf ^ 1);
-        __syncthreads();
     }
+    // The clamped tail loads must land before the next strip reuses the ring.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
 
     // Flush this wavefront's per-row counts and staged band records.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

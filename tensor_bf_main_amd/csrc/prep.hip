@@ -125,15 +125,15 @@ __global__ void k_maxabs(PassArgs a)
     }
 }
 
-// sigma = 2^ceil(log2(max|x - mu|)) so the centred, scaled rows lie in [-1, 1].
-// FP64 keeps sigma = 1 (no cast, nothing to protect).
+// sigma = 2^ceil(log2(max|x - mu|)) so the centred, scaled rows lie in [-1, 1]
+// for every dtype: the FP16/FP8 ranges are respected and no norm can overflow.
 __global__ void k_sigma(PassArgs a)
 {
     const int si = blockIdx.x * blockDim.x + threadIdx.x;
     if (si >= a.n_sets) return;
     const double m = __longlong_as_double((long long)a.scal[si].maxabs_bits);
     double sigma = 1.0;
-    if (a.dtype != BFJ_FP64 && m > 0.0 && isfinite(m)) {
+    if (m > 0.0 && isfinite(m)) {
         int e;
         frexp(m, &e); // m in [2^(e-1), 2^e)
         sigma = ldexp(1.0, e);
@@ -256,7 +256,8 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_quantize(PassArgs a)
             const int64_t row = s.row_off + i;
             // Radius: measured cast error, plus the FP64 rounding of (x - mu), inflated
             // to cover the FP64 evaluation of the sums.
-            const double q = real ? (sqrt(sq_e) + 2.3e-16 * sqrt(sq_y)) * (1.0 + 1e-9) : 0.0;
+            // (+ 1e-300 covers underflow of the scaled values, which are at most 1.)
+            const double q = real ? (sqrt(sq_e) + 2.3e-16 * sqrt(sq_y)) * (1.0 + 1e-9) + 1e-300 : 0.0;
             a.qerr[row] = q;
             a.nrm[row] = sq_n;
             if (a.dtype == BFJ_FP8) a.scales[row] = (uint8_t)scale_byte;
@@ -327,33 +328,36 @@ __global__ void k_thresholds(PassArgs a)
         const double u = fp64 ? 0x1p-53 : 0x1p-24;
         const double rho = 1.01 * (s.d + 2) * 0x1p-53;
         const double u64 = 0x1p-53;
+        // Every eps is one of: none (T < 0: no pair), all (T at least the largest
+        // possible FP64 distance of the dataset: every pair, counted in k_finalize),
+        // or banded.  Only banded eps shape g, so Z never saturates.
+        // Largest oracle d2 in kernel space: ||y_i - y_j|| <= 2 (sqrt(N_max) + q_max).
+        const double dmax = 2.0 * (sqrt(nmax) + qmax) * (1.0 + 0x1p-40);
+        const double Dmax2 = dmax * dmax * (1.0 + rho) * (1.0 + 0x1p-40);
         double Hc[BFJ_MAXEPS], Lc[BFJ_MAXEPS];
+        uint32_t all_mask = 0, none_mask = 0;
         double Hmax = -INFINITY;
-        bool any_inf = false;
         for (int e = 0; e < s.n_eps; ++e) {
             const double T = s.T[e] / sigma2; // kernel space (exact power-of-two scaling)
+            Hc[e] = Lc[e] = -INFINITY;
             if (T < 0.0) {
-                Hc[e] = -INFINITY;
-                Lc[e] = -INFINITY;
-            } else if (isinf(T)) {
-                Hc[e] = INFINITY;
-                Lc[e] = INFINITY;
-                any_inf = true;
+                none_mask |= 1u << e;
+            } else if (T >= Dmax2) {
+                all_mask |= 1u << e;
             } else {
                 const double slo = sqrt(T / (1.0 + rho)) * (1.0 - 0x1p-50);
                 const double shi = sqrt(T / (1.0 - rho)) * (1.0 + 0x1p-50);
                 Lc[e] = slo > r ? (slo - r) * (slo - r) * (1.0 - 0x1p-49) : -INFINITY;
                 Hc[e] = (shi + r) * (shi + r) * (1.0 + 0x1p-49);
                 if (query) halfband = fmax(halfband, 0.5 * (Hc[e] - (Lc[e] > 0.0 ? Lc[e] : 0.0)) * sigma2);
+                Hmax = fmax(Hmax, Hc[e]);
             }
-            Hmax = fmax(Hmax, Hc[e]);
         }
+        if (i == 0 && all_mask) atomicOr(&a.scal[si].all_mask, (unsigned long long)all_mask);
         double g; // FP64 value of the per-row initial term actually stored
         double E = 0.0;
         if (!query || Hmax == -INFINITY) {
             g = fp64 ? -1e300 : -1e30;
-        } else if (any_inf || Hmax > 1e30) {
-            g = fp64 ? 1e300 : 1e30;
         } else {
             // Accumulation bound; |g| is bounded from the previous estimate.
             const double Kp = 2.0 * K + 8.0;
@@ -365,7 +369,10 @@ __global__ void k_thresholds(PassArgs a)
                 gabs = 0.5 * fabs(Hmax - Ni) + E;
             }
             E *= 1.01;
-            const double gv = 0.5 * (Hmax + E - Ni) * (1.0 + 0x1p-50) + (Hmax + E - Ni < 0.0 ? -0x1p-1000 : 0x1p-1000);
+            // Rounded up whatever its sign: g >= 0.5 (H_max + E - N_i) makes Z < 0 a
+            // certain reject for every banded eps.
+            const double x = 0.5 * (Hmax + E - Ni);
+            const double gv = x + fabs(x) * 0x1p-50 + 0x1p-1000;
             g = fp64 ? gv : (double)f32_up(gv);
         }
         if (fp64) ((double *)a.gq)[row] = g;
@@ -374,11 +381,8 @@ __global__ void k_thresholds(PassArgs a)
         for (int e = 0; e < s.n_eps; ++e) {
             double o, v;
             if (!query || Hc[e] == -INFINITY) {
-                o = INFINITY; // everything out
+                o = INFINITY; // nothing counted here: "none" eps, or "all" eps (k_finalize)
                 v = INFINITY;
-            } else if (Hc[e] == INFINITY) {
-                o = -INFINITY;
-                v = -INFINITY; // everything in
             } else {
                 o = g + 0.5 * (Ni - E - Hc[e]);
                 v = g + 0.5 * (Ni + E - Lc[e]); // +inf when Lc = -inf

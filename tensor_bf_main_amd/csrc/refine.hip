@@ -116,7 +116,10 @@ __global__ void k_finalize(PassArgs a)
     for (int e = 0; e < s.n_eps; ++e) {
         unsigned long long c = 0;
         if (query) {
-            c = a.cnt[s.cnt_off + (int64_t)e * s.npad + i];
+            // "all" eps: every candidate, self included, is within eps (bounded in
+            // k_thresholds); the join did not count them.
+            const bool all = (a.scal[si].all_mask >> e) & 1ull;
+            c = all ? (unsigned long long)s.n : a.cnt[s.cnt_off + (int64_t)e * s.npad + i];
             const long long self = (s.T[e] >= 0.0) ? 1 : 0; // d2(i, i) == 0 exactly
             if (s.out) s.out[(int64_t)e * rows + (i - s.row_begin)] = (long long)c - self;
         }
