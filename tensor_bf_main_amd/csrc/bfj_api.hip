@@ -16,6 +16,7 @@
 #include "../../include/bfjoin.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -100,7 +101,7 @@ int bfj_join_max_blocks_per_cu(int dtype, int kc, int rows, int lds_bytes)
     return 0;
 }
 
-static const int kRowOptions[] = {16, 32, 64, 128};
+static const int kRowOptions[] = {16, 32, 64, 96, 128};
 
 // Smallest built K-chunk count >= need for this dtype (0 if none).
 static int pick_kc(int dtype, int d)
@@ -173,14 +174,19 @@ struct bfj_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
     DevBuf pack, nq, gq, ov, hmax, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
-        outbuf;
+        outbuf, cand, chunk_fill;
     uint32_t band_capacity = 0;
+    uint32_t chunk_capacity = 0; // candidate chunks of BFJ_CHUNK records
     int tile_rows[4][33] = {}; // [dtype][kc] forced/tuned R (0 = heuristic)
 };
+
+// Bumped whenever a device buffer moves: a captured graph holds raw pointers.
+static std::atomic<uint64_t> g_buf_gen{1};
 
 static int ensure(DevBuf &b, size_t bytes)
 {
     if (bytes <= b.cap) return BFJ_OK;
+    g_buf_gen.fetch_add(1);
     if (b.p) HC(hipFree(b.p));
     b.p = nullptr;
     b.cap = 0;
@@ -352,6 +358,14 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
         ctx->band_capacity = (uint32_t)want;
     }
     RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+    if (ctx->chunk_capacity == 0) {
+        // Start at 32 candidates per row (and at least 4 chunks per join block of the
+        // GPU); a pass that needs more is repeated with the size it reported.
+        const uint64_t want = std::max<uint64_t>((uint64_t)p.rows_total * 32 / BFJ_CHUNK, 4096);
+        ctx->chunk_capacity = (uint32_t)std::min<uint64_t>(want, 1u << 24);
+    }
+    RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
+    RC(ensure(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
     RC(ensure(ctx->counters, 64 * 4));
     RC(ensure(ctx->colsum, std::max<size_t>(p.blk_set.size(), 1) * BFJ_MAXDIM * 8));
     RC(ensure(ctx->mu, p.sets.size() * BFJ_MAXDIM * 8));
@@ -403,6 +417,10 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.band = (BandEntry *)ctx->band.p;
     a.band_count = (uint32_t *)ctx->counters.p;
     a.band_capacity = ctx->band_capacity;
+    a.cand = (const Candidate *)ctx->cand.p;
+    a.chunk_fill = (const uint32_t *)ctx->chunk_fill.p;
+    a.chunk_count = (const uint32_t *)ctx->counters.p + 1;
+    a.chunk_capacity = ctx->chunk_capacity;
     return a;
 }
 
@@ -431,12 +449,11 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
         j.pack = a.pack;
         j.nq = a.nq;
         j.gq = a.gq;
-        j.ov = a.ov;
         j.scales = a.scales;
-        j.cnt = a.cnt;
-        j.band = a.band;
-        j.band_count = a.band_count;
-        j.band_capacity = a.band_capacity;
+        j.cand = (Candidate *)ctx->cand.p;
+        j.chunk_fill = (uint32_t *)ctx->chunk_fill.p;
+        j.chunk_count = (uint32_t *)ctx->counters.p + 1;
+        j.chunk_capacity = ctx->chunk_capacity;
         JoinLaunch L;
         L.dtype = p.dtype;
         L.kc = g.kc;
@@ -451,6 +468,7 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
         if (rc) return fail(BFJ_ERR_DEVICE, "join launch failed: %s", hipGetErrorString(hipGetLastError()));
     }
     if (timed) HC(hipEventRecord(ctx->ev[2], st));
+    bfj_launch_classify(a, st);
     bfj_launch_refine(a, st);
     bfj_launch_finalize(a, st);
     HC(hipGetLastError());
@@ -473,18 +491,42 @@ static int check_finite(const std::vector<SetScalars> &sc)
     return BFJ_OK;
 }
 
-// Run a staged pass to completion, growing the refine list and repeating the
-// pass when it overflowed.  Totals per set land in totals_out[set][eps].
+// After a pass: if the candidate stream or the refine list overflowed (records
+// past the capacity were dropped), grow both to what the pass asked for and set
+// *again so that the caller repeats the pass.
+static int grow_on_overflow(bfj_ctx *ctx, const uint32_t ctr[2], bool *again)
+{
+    *again = false;
+    const uint32_t band_count = ctr[0], chunks = ctr[1];
+    if (chunks > ctx->chunk_capacity) {
+        ctx->chunk_capacity = (uint32_t)std::min<uint64_t>((uint64_t)chunks + chunks / 4 + 64, 1u << 26);
+        RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
+        RC(ensure(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
+        *again = true;
+    }
+    if (band_count > ctx->band_capacity) {
+        ctx->band_capacity = (uint32_t)std::min<uint64_t>((uint64_t)band_count + band_count / 4 + 1024, 0xffffffffull);
+        RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+        *again = true;
+    }
+    return BFJ_OK;
+}
+
+// Run a staged pass to completion, growing the candidate stream / refine list and
+// repeating the pass when one overflowed.  Totals per set land in totals_out[set][eps].
 static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>> *totals_out, bfj_stats *stats,
                     bool timed = true)
 {
     int retries = 0;
     for (;;) {
         RC(enqueue_pass(ctx, p, timed));
-        uint32_t band_count = 0;
-        HC(hipMemcpyAsync(&band_count, ctx->counters.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        uint32_t ctr[2] = {0, 0};
+        HC(hipMemcpyAsync(ctr, ctx->counters.p, 8, hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
-        if (band_count <= ctx->band_capacity) {
+        const uint32_t band_count = ctr[0];
+        bool again = false;
+        RC(grow_on_overflow(ctx, ctr, &again));
+        if (!again) {
             std::vector<SetScalars> sc(p.sets.size());
             HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
             RC(check_finite(sc));
@@ -520,11 +562,9 @@ static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>
             }
             return BFJ_OK;
         }
-        // Overflow: every pair past the capacity was dropped; grow and repeat the pass.
-        if (++retries > 3) return fail(BFJ_ERR_INTERNAL, "refine list keeps overflowing (%u entries)", band_count);
-        const uint64_t want = std::min<uint64_t>((uint64_t)band_count + band_count / 4 + 1024, 0xffffffffull);
-        ctx->band_capacity = (uint32_t)want;
-        RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+        if (++retries > 4)
+            return fail(BFJ_ERR_INTERNAL, "candidate stream / refine list keep overflowing (%u chunks, %u records)",
+                        ctr[1], ctr[0]);
     }
 }
 
@@ -596,7 +636,7 @@ int bfj_destroy(bfj_ctx *c)
     (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->ov, &c->hmax, &c->qerr, &c->nrm, &c->scales, &c->cnt,
                       &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
-                      &c->xcopy, &c->outbuf};
+                      &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -749,7 +789,7 @@ struct bfj_plan {
     std::vector<bfj_dataset> sets;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    uint32_t captured_capacity = 0;
+    uint64_t captured_gen = 0; // buffer generation the graph was captured with
 };
 
 static int capture(bfj_plan *pl)
@@ -770,7 +810,7 @@ static int capture(bfj_plan *pl)
     if (e != hipSuccess) return fail(BFJ_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
     pl->graph = g;
     HC(hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0));
-    pl->captured_capacity = ctx->band_capacity;
+    pl->captured_gen = g_buf_gen.load();
     return BFJ_OK;
 }
 
@@ -822,13 +862,16 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
     HC(hipSetDevice(ctx->device));
     // The context's buffers are shared with eager calls: restage if they moved.
     RC(stage_plan(ctx, pl->pass));
-    if (pl->captured_capacity != ctx->band_capacity) RC(capture(pl));
-    for (int attempt = 0; attempt < 4; ++attempt) {
+    if (pl->captured_gen != g_buf_gen.load()) RC(capture(pl));
+    for (int attempt = 0; attempt < 5; ++attempt) {
         HC(hipGraphLaunch(pl->exec, ctx->stream));
-        uint32_t band_count = 0;
-        HC(hipMemcpyAsync(&band_count, ctx->counters.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        uint32_t ctr[2] = {0, 0};
+        HC(hipMemcpyAsync(ctr, ctx->counters.p, 8, hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
-        if (band_count <= ctx->band_capacity) {
+        const uint32_t band_count = ctr[0];
+        bool again = false;
+        RC(grow_on_overflow(ctx, ctr, &again));
+        if (!again) {
             std::vector<SetScalars> sc(pl->pass.sets.size());
             HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
             RC(check_finite(sc));
@@ -860,11 +903,10 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
             }
             return BFJ_OK;
         }
-        ctx->band_capacity = (uint32_t)std::min<uint64_t>((uint64_t)band_count + band_count / 4 + 1024, 0xffffffffull);
-        RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+        RC(stage_plan(ctx, pl->pass));
         RC(capture(pl));
     }
-    return fail(BFJ_ERR_INTERNAL, "refine list keeps overflowing");
+    return fail(BFJ_ERR_INTERNAL, "candidate stream / refine list keep overflowing");
 }
 
 int bfj_plan_destroy(bfj_plan *pl)

@@ -77,6 +77,16 @@ struct WorkItem {
     int32_t rb0; // first 16-row block of the strip
 };
 
+// A pair the join could not reject (Z >= 0 for the widest banded eps).  The join
+// streams these into per-wavefront chunks; k_classify applies the per-eps tests.
+struct Candidate {
+    uint32_t row; // global row (row_off + dataset-local row)
+    uint32_t col; // dataset-local candidate row
+    double z;     // Z of the pair (kernel space)
+};
+
+#define BFJ_CHUNK 1024 // candidate records per chunk
+
 struct JoinArgs {
     const SetDesc *sets;
     const WorkItem *items;
@@ -85,12 +95,11 @@ struct JoinArgs {
     const unsigned char *pack;
     const void *nq;          // float or double per row
     const void *gq;          // float or double per row
-    const void *ov;          // float2 or double2 per (row, eps)
     const uint8_t *scales;   // FP8 per-row E8M0 scale bytes (NULL otherwise)
-    uint32_t *cnt;
-    BandEntry *band;
-    uint32_t *band_count;
-    uint32_t band_capacity;
+    Candidate *cand;         // candidate stream
+    uint32_t *chunk_fill;    // records written into each chunk
+    uint32_t *chunk_count;   // chunks handed out (may exceed chunk_capacity: overflow)
+    uint32_t chunk_capacity;
 };
 
 // Host-side launchers (one translation unit per dtype; see join_*.hip).
@@ -134,6 +143,10 @@ struct PassArgs {
     BandEntry *band;
     uint32_t *band_count;
     uint32_t band_capacity;
+    const Candidate *cand;
+    const uint32_t *chunk_fill;
+    const uint32_t *chunk_count;
+    uint32_t chunk_capacity;
 };
 
 #define BFJ_MAXDIM 256
@@ -141,5 +154,6 @@ struct PassArgs {
 
 void bfj_launch_zero(void *p, size_t bytes, hipStream_t s);
 void bfj_launch_prep(const PassArgs &a, hipStream_t s);
+void bfj_launch_classify(const PassArgs &a, hipStream_t s);
 void bfj_launch_refine(const PassArgs &a, hipStream_t s);
 void bfj_launch_finalize(const PassArgs &a, hipStream_t s);

@@ -87,7 +87,6 @@ template <> struct Pair2<double> { typedef double2 T; };
 __device__ inline int bfj_lane() { return __lane_id(); }
 
 #define BFJ_JWAVES (BFJ_BLOCK / BFJ_WAVE) // wavefronts per join block
-#define BFJ_STAGE 128                     // band records staged per wavefront
 
 // LDS layout of one join block (bytes), shared by the kernel and the host.
 template <class Tr, int KC, int R>
@@ -107,73 +106,65 @@ struct JoinLds {
     static constexpr int LPT = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0); // DMA instructions per tile
     static constexpr int RING = NB * BUF;
     static constexpr int RB = R / 16;
-    // per wavefront
-    static constexpr int ZT = RB * BFJ_WAVE * 4 * (int)sizeof(val_t);   // parked hot tiles
-    static constexpr int STG = BFJ_STAGE * (int)sizeof(BandEntry) + 16; // staged band records + count
-    __host__ __device__ static constexpr int cnt_bytes(int n_eps) { return n_eps * R * 4; }
-    __host__ __device__ static constexpr int thr_bytes(int n_eps) { return n_eps * R * 2 * (int)sizeof(val_t); }
-    __host__ __device__ static constexpr int per_wave(int n_eps)
-    {
-        return ZT + STG + ((cnt_bytes(n_eps) + thr_bytes(n_eps) + 15) / 16) * 16;
-    }
-    __host__ __device__ static constexpr int total(int n_eps) { return RING + BFJ_JWAVES * per_wave(n_eps); }
+    __host__ __device__ static constexpr int total(int) { return RING; }
 };
 
-// Copy the staged band records of this wavefront to the global refine list.
-__device__ inline void flush_band(const JoinArgs &a, BandEntry *stg, uint32_t *stg_n, int lane)
+// Per-wavefront cursor into the candidate stream: chunks of BFJ_CHUNK records
+// are handed out by one atomic each; the records themselves are plain stores
+// that nothing waits on.
+struct CandCursor {
+    uint32_t chunk; // current chunk (0xffffffff: none yet)
+    uint32_t fill;  // records written into it
+};
+
+__device__ inline void cand_close(const JoinArgs &a, const CandCursor &cc, int lane)
 {
-    const uint32_t cnt = *stg_n;
-    if (cnt == 0) return;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(a.band_count, cnt);
-    base = __shfl(base, 0);
-    for (uint32_t t = lane; t < cnt; t += BFJ_WAVE)
-        if (base + t < a.band_capacity) a.band[base + t] = stg[t];
-    // Drain: the tile pipeline counts only its own loads in vmcnt.
-    __builtin_amdgcn_s_waitcnt(0);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) *stg_n = 0;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    if (cc.chunk != 0xffffffffu && cc.chunk < a.chunk_capacity && lane == 0) a.chunk_fill[cc.chunk] = cc.fill;
+}
+
+// Append the lanes with `take` set as candidate records (wave-compacted).
+__device__ inline void cand_push(const JoinArgs &a, CandCursor &cc, bool take, uint32_t row, uint32_t col, double z,
+                                 int lane)
+{
+    const unsigned long long bal = __ballot(take);
+    if (!bal) return;
+    const uint32_t nrec = (uint32_t)__popcll(bal);
+    if (cc.fill + nrec > BFJ_CHUNK) {
+        cand_close(a, cc, lane);
+        uint32_t c = 0;
+        if (lane == 0) c = atomicAdd(a.chunk_count, 1u);
+        cc.chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(c, 0));
+        cc.fill = 0;
+    }
+    if (take && cc.chunk < a.chunk_capacity) {
+        Candidate rec;
+        rec.row = row;
+        rec.col = col;
+        rec.z = z;
+        a.cand[(size_t)cc.chunk * BFJ_CHUNK + cc.fill + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = rec;
+    }
+    cc.fill += nrec;
 }
 
 template <class Tr, int KC, int R>
-__device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, int rb0, unsigned char *lds)
+__device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, int rb0, unsigned char *lds,
+                                           CandCursor &cc)
 {
     typedef typename Tr::frag_t frag_t;
     typedef typename Tr::acc_t acc_t;
     typedef typename Tr::val_t val_t;
-    typedef typename Pair2<val_t>::T pair_t;
     typedef JoinLds<Tr, KC, R> L;
     constexpr int RB = R / 16;
     const int lane = bfj_lane();
     const int wave = threadIdx.x / BFJ_WAVE;
-    const int n_eps = s.n_eps;
     const int64_t nblk = s.npad / 16;
     const int64_t wrb0 = (int64_t)rb0 + wave * RB; // first row block of this wavefront
     const frag_t *P = reinterpret_cast<const frag_t *>(a.pack + s.pack_off);
     const val_t *nq = reinterpret_cast<const val_t *>(a.nq) + s.row_off;
     const val_t *gq = reinterpret_cast<const val_t *>(a.gq) + s.row_off;
-    const pair_t *ov = reinterpret_cast<const pair_t *>(a.ov) + s.ov_off;
     const uint8_t *sc = (Tr::dtype == BFJ_FP8) ? a.scales + s.row_off : nullptr;
 
-    unsigned char *mine = lds + L::RING + wave * L::per_wave(n_eps);
-    val_t *zt = reinterpret_cast<val_t *>(mine);
-    BandEntry *stg = reinterpret_cast<BandEntry *>(mine + L::ZT);
-    uint32_t *stg_n = reinterpret_cast<uint32_t *>(mine + L::ZT + BFJ_STAGE * sizeof(BandEntry));
-    uint32_t *lcnt = reinterpret_cast<uint32_t *>(mine + L::ZT + L::STG);
-    pair_t *thr = reinterpret_cast<pair_t *>(mine + L::ZT + L::STG + L::cnt_bytes(n_eps));
-
-    // Per-row state of this wavefront's rows: counters, thresholds (LDS), fragments
-    // and initial values (VGPRs).
-    for (int t = lane; t < n_eps * R; t += BFJ_WAVE) {
-        lcnt[t] = 0;
-        const int li = t % R, e = t / R;
-        const int64_t i = wrb0 * 16 + li;
-        thr[e * R + li] = (i < s.npad) ? ov[i * n_eps + e] : pair_t{(val_t)3e38, (val_t)3e38};
-    }
-    if (lane == 0) *stg_n = 0;
+    // Fragments and initial values of this wavefront's rows, resident in VGPRs.
     frag_t A[RB][KC];
     val_t G[RB][4];
     int SA[RB];
@@ -224,7 +215,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     static_assert(WAIT_N < 64, "vmcnt field");
 
     // Drain the strip's own loads here, so that inside the loop the only
-    // outstanding vector-memory operations are the ring's loads.
+    // outstanding vector-memory loads are the ring's.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 #pragma unroll
@@ -261,7 +252,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 
             // Inputs are NaN-free (the pass rejects non-finite rows), so with
             // -fno-honor-nans this is a tree of v_max3.  -0.0 >= 0 holds, so a zero Z
-            // of either sign reaches the exact tests below.
+            // of either sign is kept.
             val_t m = acc[0][0];
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
@@ -269,51 +260,20 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
                 for (int r = 0; r < 4; ++r) m = fmax(m, acc[rb][r]);
 
             if (__builtin_expect(__ballot(m >= (val_t)0) != 0ull, 0)) {
-                // Hot tiles: park them (one 16-byte LDS store per lane each) and walk
-                // them with a rolled loop, so this rare path costs no registers.
-                uint32_t hot = 0;
+                // Rare: stream every pair with Z >= 0 as a candidate record (wave
+                // compacted, plain stores); k_classify applies the per-eps tests.
+                const int64_t col = cb * 16 + (lane & 15);
+                const bool col_ok = col < s.n;
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb) {
                     const val_t mr = fmax(fmax(acc[rb][0], acc[rb][1]), fmax(acc[rb][2], acc[rb][3]));
-                    if (__ballot(mr >= (val_t)0) != 0ull) {
-                        hot |= 1u << rb;
-                        *reinterpret_cast<acc_t *>(zt + (rb * BFJ_WAVE + lane) * 4) = acc[rb];
-                    }
-                }
-                hot = (uint32_t)__builtin_amdgcn_readfirstlane((int)hot);
-                const int64_t col = cb * 16 + (lane & 15);
-                const bool col_ok = col < s.n;
+                    if (__ballot(mr >= (val_t)0 && col_ok) == 0ull) continue;
 #pragma unroll 1
-                for (int q = 0; q < RB * 4; ++q) {
-                    if (!((hot >> (q >> 2)) & 1u)) {
-                        q |= 3; // rest of this row block is cold
-                        continue;
+                    for (int r = 0; r < 4; ++r) {
+                        const val_t z = r == 0 ? acc[rb][0] : (r == 1 ? acc[rb][1] : (r == 2 ? acc[rb][2] : acc[rb][3]));
+                        const uint32_t row = (uint32_t)(s.row_off + (wrb0 + rb) * 16 + Tr::out_row(lane, r));
+                        cand_push(a, cc, z >= (val_t)0 && col_ok, row, (uint32_t)col, (double)z, lane);
                     }
-                    const val_t z = zt[((q >> 2) * BFJ_WAVE + lane) * 4 + (q & 3)];
-                    const bool cand = z >= (val_t)0 && col_ok;
-                    if (__ballot(cand) == 0ull) continue;
-                    if (*stg_n > BFJ_STAGE - BFJ_WAVE) flush_band(a, stg, stg_n, lane);
-                    const int li = (q >> 2) * 16 + Tr::out_row(lane, q & 3);
-                    uint32_t band = 0;
-                    if (cand) {
-                        for (int e = 0; e < n_eps; ++e) {
-                            const pair_t t = thr[e * R + li];
-                            if (z >= t.y) atomicAdd(&lcnt[e * R + li], 1u);
-                            else if (z >= t.x) band |= 1u << e;
-                        }
-                        if (band) {
-                            const uint32_t idx = atomicAdd(stg_n, 1u);
-                            BandEntry be;
-                            be.i = (uint32_t)(wrb0 * 16 + li);
-                            be.j = (uint32_t)col;
-                            be.z = (float)z;
-                            be.meta = band | ((uint32_t)s.set_id << 16);
-                            stg[idx] = be;
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
         }
@@ -321,19 +281,6 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     // The clamped tail loads must land before the next strip reuses the ring.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-
-    // Flush this wavefront's per-row counts and staged band records.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int t = lane; t < n_eps * R; t += BFJ_WAVE) {
-        const uint32_t v = lcnt[t];
-        if (v) {
-            const int e = t / R, li = t % R;
-            atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + wrb0 * 16 + li], v);
-        }
-    }
-    flush_band(a, stg, stg_n, lane);
 }
 
 // Persistent kernel: every block takes 4R-row strips from a global queue until
@@ -344,6 +291,10 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_join(JoinArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     __shared__ uint32_t item_s;
+    CandCursor cc;
+    cc.chunk = 0xffffffffu;
+    cc.fill = BFJ_CHUNK;
+    const int lane = bfj_lane();
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(a.item_counter, 1u);
         __syncthreads();
@@ -352,6 +303,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_join(JoinArgs a)
         if (item >= (uint32_t)a.n_items) break;
         const WorkItem w = a.items[item];
         const SetDesc s = a.sets[w.set];
-        join_strip<Tr, KC, R>(a, s, w.rb0, lds_raw);
+        join_strip<Tr, KC, R>(a, s, w.rb0, lds_raw, cc);
     }
+    cand_close(a, cc, lane);
 }

@@ -17,6 +17,98 @@
 #include "bfj_internal.h"
 #include <math.h>
 
+// ---- classification of the join's candidate stream --------------------------------
+// One wavefront per chunk.  Per record and eps: certain-in (Z >= v) is counted,
+// inside the band (o <= Z < v) goes to the refine list, otherwise out.  Counts are
+// summed per distinct row of the wavefront before one atomic per row and eps.
+__device__ inline int set_of_row(const PassArgs &a, uint32_t row)
+{
+    int lo = 0, hi = a.n_sets - 1;
+    while (lo < hi) { // last set with row_off <= row
+        const int mid = (lo + hi + 1) >> 1;
+        if ((uint64_t)a.sets[mid].row_off <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t n_chunks = min(*a.chunk_count, a.chunk_capacity);
+    const uint32_t wave_id = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
+    const uint32_t n_waves = gridDim.x * (BFJ_BLOCK / BFJ_WAVE);
+    const bool fp64 = a.dtype == BFJ_FP64;
+    for (uint32_t ch = wave_id; ch < n_chunks; ch += n_waves) {
+        const uint32_t fill = min(a.chunk_fill[ch], (uint32_t)BFJ_CHUNK);
+        for (uint32_t t0 = 0; t0 < fill; t0 += BFJ_WAVE) {
+            const uint32_t t = t0 + lane;
+            const bool live = t < fill;
+            Candidate c;
+            c.row = 0;
+            c.col = 0;
+            c.z = -1.0;
+            if (live) c = a.cand[(size_t)ch * BFJ_CHUNK + t];
+            const int si = live ? (a.n_sets == 1 ? 0 : set_of_row(a, c.row)) : 0;
+            const SetDesc &s = a.sets[si];
+            const uint32_t i = c.row - (uint32_t)s.row_off;
+            uint32_t in_mask = 0, band = 0;
+            if (live) {
+                for (int e = 0; e < s.n_eps; ++e) {
+                    double o, v;
+                    if (fp64) {
+                        const double2 p = ((const double2 *)a.ov)[s.ov_off + (int64_t)i * s.n_eps + e];
+                        o = p.x;
+                        v = p.y;
+                    } else {
+                        const float2 p = ((const float2 *)a.ov)[s.ov_off + (int64_t)i * s.n_eps + e];
+                        o = p.x;
+                        v = p.y;
+                    }
+                    if (c.z >= v) in_mask |= 1u << e;
+                    else if (c.z >= o) band |= 1u << e;
+                }
+            }
+            // Certain-in counts, summed per distinct (row) of the wavefront.
+            unsigned long long pending = __ballot(in_mask != 0u);
+            while (pending) {
+                const int leader = __ffsll((long long)pending) - 1;
+                const uint32_t key = __shfl(c.row, leader);
+                const bool same = c.row == key && live;
+                for (int e = 0; e < BFJ_MAXEPS; ++e) {
+                    const unsigned long long m = __ballot(same && ((in_mask >> e) & 1u));
+                    if (m && lane == leader)
+                        atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + i], (uint32_t)__popcll(m));
+                }
+                pending &= ~__ballot(same);
+            }
+            // Band pairs to the refine list: one atomic per wavefront.
+            const unsigned long long bb = __ballot(band != 0u);
+            if (bb) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.band_count, (uint32_t)__popcll(bb));
+                base = __shfl(base, 0);
+                if (band) {
+                    const uint32_t idx = base + (uint32_t)__popcll(bb & ((1ull << lane) - 1ull));
+                    if (idx < a.band_capacity) {
+                        BandEntry be;
+                        be.i = i;
+                        be.j = c.col;
+                        be.z = (float)c.z;
+                        be.meta = band | ((uint32_t)si << 16);
+                        a.band[idx] = be;
+                    }
+                }
+            }
+        }
+    }
+}
+
+void bfj_launch_classify(const PassArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_classify, dim3(2048), dim3(BFJ_BLOCK), 0, st, a);
+}
+
 #define RCH 16                 // doubles per row chunk (one 128-byte line)
 #define RPAD (BFJ_WAVE + 1)    // LDS row length: 65 doubles keeps both phases conflict-free
 #define RWAVES 2              // wavefronts per block: 2 x 17 KiB of staged rows
@@ -77,10 +169,32 @@ __global__ __launch_bounds__(RWAVES * BFJ_WAVE) void k_refine(PassArgs a)
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         double err = -1.0;
+        // Hits per (dataset, row, eps): records of one group share few rows, so the
+        // lanes of each distinct (dataset, row) are summed first and one lane adds
+        // the sum (a vote loop over the distinct keys instead of contended atomics).
+        uint32_t hits = 0;
         if (live) {
             const uint32_t mask = be.meta & 0xffu;
             for (int e = 0; e < s.n_eps; ++e)
-                if (((mask >> e) & 1u) && acc <= s.T[e]) atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + be.i], 1u);
+                if (((mask >> e) & 1u) && acc <= s.T[e]) hits |= 1u << e;
+        }
+        const uint64_t key = live ? (((uint64_t)(uint32_t)si << 32) | be.i) : ~0ull;
+        unsigned long long pending = __ballot(hits != 0u);
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const uint64_t k = __shfl(key, leader);
+            const bool same = key == k;
+            const unsigned long long grp = __ballot(same && hits != 0u);
+            // Per-eps sums over the group (uniform loop: every lane takes part in the votes;
+            // the leader's own set and row address the counter).
+            for (int e = 0; e < BFJ_MAXEPS; ++e) {
+                const unsigned long long m = __ballot(same && ((hits >> e) & 1u));
+                if (m && lane == leader)
+                    atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + be.i], (uint32_t)__popcll(m));
+            }
+            pending &= ~grp;
+        }
+        if (live) {
             const double sigma = a.scal[si].sigma;
             const double approx = sigma * sigma * (a.hmax[s.row_off + be.i] - 2.0 * (double)be.z);
             err = fabs(approx - acc);
