@@ -19,6 +19,7 @@ Other workloads (one JSON line each, same keys):
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3] [--n N] ...
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -180,6 +181,26 @@ def batch_workload(args, ctx):
     return out
 
 
+def copy_calibration(ctx, mib):
+    """One hipMemcpy of a known size through the HIP runtime the library uses:
+    the FETCH_SIZE / WRITE_SIZE counters it produces convert counter readings of
+    the join into HBM bytes."""
+    import tensor_bf_main_amd as bfj
+    nbytes = mib << 20
+    src = bfj.DeviceArray(ctx, (nbytes,), np.uint8)
+    dst = bfj.DeviceArray(ctx, (nbytes,), np.uint8)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipDeviceSynchronize.argtypes = []
+    rc = hip.hipMemcpy(dst.data_ptr(), src.data_ptr(), nbytes, 3)  # hipMemcpyDeviceToDevice
+    rc = rc or hip.hipDeviceSynchronize()
+    if rc:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+    print(json.dumps({"copy_bytes": nbytes}), flush=True)
+    src.free()
+    dst.free()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +214,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0, help="query rows timed on the CPU (0 = auto, ~10-30 s)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (scales HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the join runs on a single MI355X in this build (asked for {args.gpus}).",
@@ -203,6 +226,10 @@ def main():
     from tensor_bf_main_amd import workloads as wl
 
     ctx = bfj.Context(0)
+    if args.copy_calibration:
+        copy_calibration(ctx, args.copy_calibration)
+        ctx.close()
+        return
     if args.workload == 5:
         args.dtype = args.dtype or "fp16"
         out = batch_workload(args, ctx)

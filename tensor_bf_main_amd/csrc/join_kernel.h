@@ -8,22 +8,24 @@
 //
 //   * A block of 4 wavefronts owns a strip of 4R query rows; each wavefront
 //     keeps the operand fragments of its R rows in VGPRs for the whole pass.
-//   * The block streams every candidate block of 16 rows through a
-//     double-buffered LDS tile: one coalesced load per block instead of one per
-//     wavefront (the reference reads candidates from global memory in every
-//     warp, kernel_join.cu:432), one barrier per candidate block.
+//   * The block streams every candidate block of 16 rows through an NB-deep
+//     LDS ring filled by direct-to-LDS loads (global_load_lds): one coalesced
+//     load per block instead of one per wavefront (the reference reads
+//     candidates from global memory in every warp, kernel_join.cu:432), no
+//     VGPRs spent on staging, one barrier per candidate block.
 //   * The accumulator of each 16x16 tile starts at C0 = g_i - n_j, so after
 //     the MFMAs Z = 0.5*(H_i - d2_approx(i, j)) with no further arithmetic: the
 //     squared norms and the threshold are folded into the initial value
 //     (the reference pre-loads candidate chunk norms the same way,
 //     kernel_join.cu:433-437).
-//   * Z < 0 means "certainly out".  One max tree and one wave vote skip almost
-//     every tile.  Hot tiles are parked in LDS and tested exactly against the
-//     per-row thresholds, also held in LDS; certain-in pairs bump per-row LDS
-//     counters, band pairs are staged in LDS and flushed to the refine list 64
-//     at a time.  The hot path never waits on global memory.
-//   * Per-row counts are flushed with one global atomic per row per strip,
-//     instead of one atomic per hit (kernel_join.cu:484, :662).
+//   * Z < 0 means "certainly out" for every epsilon.  One max tree and one wave
+//     vote skip almost every tile.  The pairs of a hot tile with Z >= 0 are
+//     appended to a candidate stream (row, col, Z) with wave-compacted plain
+//     stores into per-wavefront chunks of 1024 records (one atomic per chunk),
+//     so the hot path never waits on memory or on another wavefront; the
+//     per-epsilon tests, counting and band selection run afterwards in
+//     k_classify (refine.hip).  The reference instead does one global atomic
+//     per hit inside the join (kernel_join.cu:484, :662).
 #pragma once
 #include "bfj_internal.h"
 
@@ -79,10 +81,6 @@ struct TrF64 {
     // The FP64 shape stores tile row 4*r + l/16 in register r of lane l.
     __device__ static inline int out_row(int lane, int r) { return 4 * r + (lane >> 4); }
 };
-
-template <class V> struct Pair2;
-template <> struct Pair2<float> { typedef float2 T; };
-template <> struct Pair2<double> { typedef double2 T; };
 
 __device__ inline int bfj_lane() { return __lane_id(); }
 

@@ -335,13 +335,13 @@ __global__ void k_thresholds(PassArgs a)
         const double dmax = 2.0 * (sqrt(nmax) + qmax) * (1.0 + 0x1p-40);
         const double Dmax2 = dmax * dmax * (1.0 + rho) * (1.0 + 0x1p-40);
         double Hc[BFJ_MAXEPS], Lc[BFJ_MAXEPS];
-        uint32_t all_mask = 0, none_mask = 0;
+        uint32_t all_mask = 0;
         double Hmax = -INFINITY;
         for (int e = 0; e < s.n_eps; ++e) {
             const double T = s.T[e] / sigma2; // kernel space (exact power-of-two scaling)
             Hc[e] = Lc[e] = -INFINITY;
             if (T < 0.0) {
-                none_mask |= 1u << e;
+                // no pair: both thresholds stay -inf
             } else if (T >= Dmax2) {
                 all_mask |= 1u << e;
             } else {

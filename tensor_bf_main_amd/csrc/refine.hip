@@ -1,4 +1,9 @@
-// refine.hip -- FP64 band refine and count finalisation.
+// refine.hip -- classification of the join's candidates, FP64 band refine and
+// count finalisation.
+//
+// k_classify reads the candidate stream the join wrote (pairs with Z >= 0) and
+// applies the per-epsilon tests of the error bound: certain-in pairs are counted,
+// pairs inside the band go to the refine list.
 //
 // Every pair the join kernel could not decide is re-evaluated here in the
 // reference's own arithmetic: the direct form of distanceCalculationBruteForceCuda
