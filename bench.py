@@ -97,7 +97,9 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
         "stage_ms": {k: last.stats[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
         "tile_rows": last.stats["tile_rows"],
     }
-    flops = 2.0 * d * pairs  # algorithmic: true d, FMA = 2
+    # Algorithmic flops: true d, FMA = 2, one dot product per unordered pair (the
+    # symmetric sweep decides both (i, j) and (j, i) from it).
+    flops = 2.0 * d * n * (n + 1) / 2
     out["roofline"] = roofline(flops, float(np.mean(join_ms)), dtype)
     got = c_main.numpy()
     mismatches = None
@@ -133,7 +135,7 @@ def batch_workload(args, ctx):
     dev = [{"data": ctx.to_device(s["data"]), "eps": s["eps"],
             "counts": ctx.empty((len(s["eps"]), s["data"].shape[0]), np.int64)} for s in sets]
     pairs = float(sum(s["data"].shape[0] ** 2 for s in sets))
-    flops = float(sum(2.0 * s["data"].shape[1] * s["data"].shape[0] ** 2 for s in sets))
+    flops = float(sum(s["data"].shape[1] * s["data"].shape[0] * (s["data"].shape[0] + 1.0) for s in sets))
     log("plan: autotune + capture")
     t0 = time.perf_counter()
     plan = ctx.plan(dev, dtype=args.dtype)

@@ -173,7 +173,7 @@ struct bfj_ctx {
     int n_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
-    DevBuf pack, nq, gq, ov, hmax, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
+    DevBuf pack, nq, gq, lh, he, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
         outbuf, cand, chunk_fill;
     uint32_t band_capacity = 0;
     uint32_t chunk_capacity = 0; // candidate chunks of BFJ_CHUNK records
@@ -231,7 +231,7 @@ struct PassPlan {
     std::vector<Group> groups;
     std::vector<int32_t> blk_set, set_blk0, set_nblk;
     std::vector<int64_t> blk_row0;
-    int64_t rows_total = 0, pack_bytes = 0, ov_total = 0, cnt_total = 0;
+    int64_t rows_total = 0, pack_bytes = 0, lh_total = 0, cnt_total = 0;
     uint64_t pairs = 0;
     int max_rows = 0, max_kc = 0;
 };
@@ -275,15 +275,16 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         s.kc = pick_kc(dtype, h.d);
         s.n_eps = h.n_eps;
         s.set_id = (int32_t)si;
+        s.sym = (h.row_begin == 0 && h.row_end == h.n) ? 1 : 0;
         s.out = h.out;
         for (int e = 0; e < h.n_eps; ++e) s.T[e] = eps_threshold(h.eps[e]);
         s.row_off = p.rows_total;
         s.pack_off = p.pack_bytes;
-        s.ov_off = p.ov_total;
+        s.lh_off = p.lh_total;
         s.cnt_off = p.cnt_total;
         p.rows_total += s.npad;
         p.pack_bytes += (s.npad / 16) * (int64_t)s.kc * BFJ_WAVE * FB;
-        p.ov_total += s.npad * s.n_eps;
+        p.lh_total += s.npad * s.n_eps;
         p.cnt_total += s.npad * s.n_eps;
         p.pairs += (uint64_t)(h.row_end - h.row_begin) * (uint64_t)h.n;
         p.max_kc = std::max(p.max_kc, s.kc);
@@ -320,12 +321,13 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
             const SetDesc &s = p.sets[si];
             if (s.kc != kc || s.row_end <= s.row_begin) continue;
             const int64_t rb_begin = s.row_begin / 16, rb_end = (s.row_end + 15) / 16;
-            // One item = one block strip: 4 wavefronts x R rows.
+            // One item = one block strip: 4 wavefronts x R rows, sweeping all candidate
+            // blocks, or (symmetric sweep) those from its first block on.
             for (int64_t rb = rb_begin; rb < rb_end; rb += (BFJ_BLOCK / BFJ_WAVE) * rows / 16) {
                 WorkItem w;
                 w.set = (int32_t)si;
                 w.rb0 = (int32_t)rb;
-                its.push_back({s.npad, w});
+                its.push_back({s.npad / 16 - (s.sym ? rb : 0), w});
             }
         }
         std::stable_sort(its.begin(), its.end(), [](const std::pair<int64_t, WorkItem> &a,
@@ -347,8 +349,8 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->pack, std::max<int64_t>(p.pack_bytes, 64)));
     RC(ensure(ctx->nq, nrows * vb));
     RC(ensure(ctx->gq, nrows * vb));
-    RC(ensure(ctx->ov, std::max<int64_t>(p.ov_total, 1) * 2 * vb));
-    RC(ensure(ctx->hmax, nrows * 8));
+    RC(ensure(ctx->lh, std::max<int64_t>(p.lh_total, 1) * 16));
+    RC(ensure(ctx->he, nrows * 16));
     RC(ensure(ctx->qerr, nrows * 8));
     RC(ensure(ctx->nrm, nrows * 8));
     RC(ensure(ctx->scales, nrows));
@@ -407,8 +409,8 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.pack = (unsigned char *)ctx->pack.p;
     a.nq = ctx->nq.p;
     a.gq = ctx->gq.p;
-    a.ov = ctx->ov.p;
-    a.hmax = (double *)ctx->hmax.p;
+    a.lh = (double2 *)ctx->lh.p;
+    a.he = (double2 *)ctx->he.p;
     a.qerr = (double *)ctx->qerr.p;
     a.nrm = (double *)ctx->nrm.p;
     a.scales = (uint8_t *)ctx->scales.p;
@@ -634,7 +636,7 @@ int bfj_destroy(bfj_ctx *c)
     if (!c) return BFJ_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->ov, &c->hmax, &c->qerr, &c->nrm, &c->scales, &c->cnt,
+    DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->lh, &c->he, &c->qerr, &c->nrm, &c->scales, &c->cnt,
                       &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
                       &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill};
     for (DevBuf *b : bufs) release(*b);

@@ -7,9 +7,11 @@
 //              rows and every K-chunk, 64 lanes x FRAG_BYTES contiguous, so one
 //              wavefront load of a fragment is a single coalesced 1-2 KiB read
 //   nq[row]    0.5 * ||xhat_j||^2 (FP32 or FP64), +BIG for padding rows
-//   gq[row]    0.5 * (H_i - N_i) rounded up; -BIG for rows that are not queries
-//   ov[row][e] certain-out / certain-in thresholds on Z for each epsilon
-//   hmax[row]  H_i (FP64), used to report the approximation error of refined pairs
+//   gq[row]    g_i = 0.5 * (H* + E_i - N_i) rounded up (H*: largest banded H of the
+//              dataset); -BIG for rows that are not queries
+//   lh[row][e] (L, H): squared-distance bounds (kernel space) of row i for each
+//              epsilon: S <= L is certainly within eps, S > H certainly not
+//   he[row]    (2 g_i + N_i, E_i): S_hat = he.x - 2 Z estimates S with |S_hat - S| <= E_i
 //   cnt[e][row] per-row hit counts, self included (uint32)
 //   band list  (i, j, z, mask) records of pairs whose band straddles a threshold
 #pragma once
@@ -40,13 +42,14 @@ struct SetDesc {
     int64_t row_end;
     int64_t row_off;   // offset in the row-indexed arrays
     int64_t pack_off;  // byte offset of this dataset's fragments in pack
-    int64_t ov_off;    // offset (in threshold pairs) into ov
+    int64_t lh_off;    // offset (in bound pairs) into lh
     int64_t cnt_off;   // offset into cnt; cnt[cnt_off + e * npad + i]
     int64_t *out;      // device int64 counts [n_eps][row_end-row_begin], self excluded (may be NULL)
     int32_t d;         // true dimension
     int32_t kc;        // K-chunks after padding
     int32_t n_eps;
     int32_t set_id;
+    int32_t sym;       // every row is a query: each unordered pair is evaluated once
     double T[BFJ_MAXEPS]; // input-space thresholds: hit iff d2 <= T
 };
 
@@ -65,10 +68,10 @@ struct SetScalars {
 
 // A pair whose certain-in / certain-out tests were both inconclusive.
 struct BandEntry {
-    uint32_t i;     // dataset-local query row
+    uint32_t i;     // dataset-local row of the tile (the one Z was computed for)
     uint32_t j;     // dataset-local candidate row
     float z;        // Z value of the pair from the join kernel (kernel space)
-    uint32_t meta;  // bits 0..7: eps mask; bits 16..31: set index
+    uint32_t meta;  // bits 0..7: eps to decide for row i; 8..15: for row j; 16..31: set index
 };
 
 // A join work item: one strip of R query rows of one dataset.
@@ -81,9 +84,10 @@ struct WorkItem {
 // streams these into per-wavefront chunks; k_classify applies the per-eps tests.
 struct Candidate {
     uint32_t row; // global row (row_off + dataset-local row)
-    uint32_t col; // dataset-local candidate row
+    uint32_t col; // bits 0..30: dataset-local candidate row; bit 31: the pair counts for it too
     double z;     // Z of the pair (kernel space)
 };
+#define BFJ_COL_SYM 0x80000000u
 
 #define BFJ_CHUNK 1024 // candidate records per chunk
 
@@ -134,8 +138,9 @@ struct PassArgs {
     double *colsum;            // [row_blocks][BFJ_MAXDIM] partial column sums
     double *mu;                // [n_sets][BFJ_MAXDIM]
     unsigned char *pack;
-    void *nq, *gq, *ov;
-    double *hmax;
+    void *nq, *gq;
+    double2 *lh;               // [lh_off + i * n_eps + e], i dataset-local
+    double2 *he;               // per row
     double *qerr, *nrm;        // per row
     uint8_t *scales;
     uint32_t *cnt;

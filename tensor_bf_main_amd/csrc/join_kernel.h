@@ -216,9 +216,16 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     // outstanding vector-memory loads are the ring's.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    // A dataset whose rows are all queries sweeps only candidate blocks from the
+    // strip's first block on: every unordered pair lies in exactly one tile (rb, cb)
+    // with cb >= rb, and that tile decides it for both rows.
+    const int64_t cb0 = s.sym ? (int64_t)rb0 : 0;
 #pragma unroll
-    for (int t = 0; t < L::NB - 1; ++t) issue(t < nblk ? t : nblk - 1, t);
-    for (int64_t cb = 0; cb < nblk; ++cb) {
+    for (int t = 0; t < L::NB - 1; ++t) {
+        const int64_t tt = cb0 + t;
+        issue(tt < nblk ? tt : nblk - 1, (int)(tt % L::NB));
+    }
+    for (int64_t cb = cb0; cb < nblk; ++cb) {
         __builtin_amdgcn_s_waitcnt(WAIT_IMM);
         // All quarters of tile cb landed (each wavefront waited for its own loads
         // above); every wavefront is done with tile cb-1, whose fragments fed its
@@ -260,17 +267,24 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             if (__builtin_expect(__ballot(m >= (val_t)0) != 0ull, 0)) {
                 // Rare: stream every pair with Z >= 0 as a candidate record (wave
                 // compacted, plain stores); k_classify applies the per-eps tests.
+                // In a symmetric sweep the tiles with cb < rb and the lower half of
+                // cb == rb belong to another tile and are dropped here.
                 const int64_t col = cb * 16 + (lane & 15);
                 const bool col_ok = col < s.n;
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb) {
+                    const int64_t blk = wrb0 + rb;
+                    if (s.sym && cb < blk) continue;
                     const val_t mr = fmax(fmax(acc[rb][0], acc[rb][1]), fmax(acc[rb][2], acc[rb][3]));
                     if (__ballot(mr >= (val_t)0 && col_ok) == 0ull) continue;
 #pragma unroll 1
                     for (int r = 0; r < 4; ++r) {
                         const val_t z = r == 0 ? acc[rb][0] : (r == 1 ? acc[rb][1] : (r == 2 ? acc[rb][2] : acc[rb][3]));
-                        const uint32_t row = (uint32_t)(s.row_off + (wrb0 + rb) * 16 + Tr::out_row(lane, r));
-                        cand_push(a, cc, z >= (val_t)0 && col_ok, row, (uint32_t)col, (double)z, lane);
+                        const int64_t li = blk * 16 + Tr::out_row(lane, r);
+                        const bool keep = col_ok && (!s.sym || col >= li);
+                        const uint32_t tag = (s.sym && col != li) ? BFJ_COL_SYM : 0u;
+                        cand_push(a, cc, z >= (val_t)0 && keep, (uint32_t)(s.row_off + li), (uint32_t)col | tag,
+                                  (double)z, lane);
                     }
                 }
             }

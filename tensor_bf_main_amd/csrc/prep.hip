@@ -10,7 +10,7 @@
 // per-row error band that makes the low-precision decisions rigorous.
 //
 // Band derivation (kernel space; the join kernel computes
-//   Z_ij = MFMA(C0 = g_i - n_j, xhat_i . xhat_j)  ~=  0.5 * (H_i - S_ij),
+//   Z_ij = MFMA(C0 = g_i - n_j, xhat_i . xhat_j)  ~=  0.5 * (2 g_i + N_i - S_ij),
 //   S_ij = ||xhat_i - xhat_j||^2):
 //   * quantisation: | ||xhat_i - xhat_j|| - ||y_i - y_j|| | <= q_i + q_j,
 //     q_i >= ||y_i - xhat_i|| measured per row, y the exact centred row.
@@ -21,11 +21,13 @@
 //     bound for K' = 2K + 8 round-to-nearest additions in any order at unit
 //     roundoff u (2^-24 for FP32 accumulation, 2^-53 for FP64), plus the
 //     rounding of C0 and of the FP32 norms.
-//   Certain-out for eps e:  Z < o_ie = g_i + 0.5 (N_i - E_i - H_e)  (rounded down)
-//   Certain-in  for eps e:  Z >= v_ie = g_i + 0.5 (N_i + E_i - L_e) (rounded up)
-//   with H_e = (sqrt(T_e/(1-rho)) + r_i)^2, L_e = (sqrt(T_e/(1+rho)) - r_i)^2,
-//   r_i = q_i + q_max, and g_i = 0.5 (H_max + E_i - N_i) rounded up, which
-//   makes o_i,max >= 0: Z < 0 is certainly out for every eps.
+//   So S_hat = (2 g_i + N_i) - 2 Z satisfies |S_hat - S_ij| <= E_i, and for a row
+//   k of the pair (k = i or j) and eps e:
+//     certainly within eps   if S_hat + E_i <= L_ke = (sqrt(T_e/(1+rho)) - r_k)^2
+//     certainly outside      if S_hat - E_i >  H_ke = (sqrt(T_e/(1-rho)) + r_k)^2
+//   with r_k = q_k + q_max.  g_i = 0.5 (H* + E_i - N_i) rounded up, H* the H of
+//   r = 2 q_max (>= every H_ke), makes Z < 0 certainly outside for every eps and
+//   both rows: one Z per unordered pair decides for i and for j.
 #include "bfj_internal.h"
 #include <math.h>
 #include <float.h>
@@ -336,12 +338,14 @@ __global__ void k_thresholds(PassArgs a)
         const double Dmax2 = dmax * dmax * (1.0 + rho) * (1.0 + 0x1p-40);
         double Hc[BFJ_MAXEPS], Lc[BFJ_MAXEPS];
         uint32_t all_mask = 0;
-        double Hmax = -INFINITY;
+        // H* bounds H of every row of the dataset (r <= 2 q_max), so that Z < 0 is a
+        // certain reject for the row of the tile and for the candidate row alike.
+        double Hstar = -INFINITY;
         for (int e = 0; e < s.n_eps; ++e) {
             const double T = s.T[e] / sigma2; // kernel space (exact power-of-two scaling)
             Hc[e] = Lc[e] = -INFINITY;
             if (T < 0.0) {
-                // no pair: both thresholds stay -inf
+                // no pair: both bounds stay -inf
             } else if (T >= Dmax2) {
                 all_mask |= 1u << e;
             } else {
@@ -350,53 +354,39 @@ __global__ void k_thresholds(PassArgs a)
                 Lc[e] = slo > r ? (slo - r) * (slo - r) * (1.0 - 0x1p-49) : -INFINITY;
                 Hc[e] = (shi + r) * (shi + r) * (1.0 + 0x1p-49);
                 if (query) halfband = fmax(halfband, 0.5 * (Hc[e] - (Lc[e] > 0.0 ? Lc[e] : 0.0)) * sigma2);
-                Hmax = fmax(Hmax, Hc[e]);
+                const double rs = 2.0 * qmax;
+                Hstar = fmax(Hstar, (shi + rs) * (shi + rs) * (1.0 + 0x1p-49));
             }
         }
         if (i == 0 && all_mask) atomicOr(&a.scal[si].all_mask, (unsigned long long)all_mask);
         double g; // FP64 value of the per-row initial term actually stored
         double E = 0.0;
-        if (!query || Hmax == -INFINITY) {
+        if (!query || Hstar == -INFINITY) {
             g = fp64 ? -1e300 : -1e30;
         } else {
             // Accumulation bound; |g| is bounded from the previous estimate.
             const double Kp = 2.0 * K + 8.0;
             const double coef = 2.0 * (1.01 * Kp + 1.0) * u;
             const double cross = sqrt(Ni * nmax);
-            double gabs = 0.5 * fabs(Hmax - Ni);
+            double gabs = 0.5 * fabs(Hstar - Ni);
             for (int it = 0; it < 2; ++it) {
                 E = coef * (gabs + nmax + cross) + 4.0 * (K + 4) * u64 * (Ni + nmax + gabs) + K * 0x1p-120;
-                gabs = 0.5 * fabs(Hmax - Ni) + E;
+                gabs = 0.5 * fabs(Hstar - Ni) + E;
             }
             E *= 1.01;
-            // Rounded up whatever its sign: g >= 0.5 (H_max + E - N_i) makes Z < 0 a
-            // certain reject for every banded eps.
-            const double x = 0.5 * (Hmax + E - Ni);
+            // Rounded up whatever its sign: 2 g + N_i - E >= H* makes Z < 0 a certain
+            // reject (S >= S_hat - E > H*) for every banded eps and both rows.
+            const double x = 0.5 * (Hstar + E - Ni);
             const double gv = x + fabs(x) * 0x1p-50 + 0x1p-1000;
             g = fp64 ? gv : (double)f32_up(gv);
         }
         if (fp64) ((double *)a.gq)[row] = g;
         else ((float *)a.gq)[row] = (float)g;
-        a.hmax[row] = 2.0 * g + Ni;
+        a.he[row] = make_double2(2.0 * g + Ni, E);
         for (int e = 0; e < s.n_eps; ++e) {
-            double o, v;
-            if (!query || Hc[e] == -INFINITY) {
-                o = INFINITY; // nothing counted here: "none" eps, or "all" eps (k_finalize)
-                v = INFINITY;
-            } else {
-                o = g + 0.5 * (Ni - E - Hc[e]);
-                v = g + 0.5 * (Ni + E - Lc[e]); // +inf when Lc = -inf
-                o = o - fabs(o) * 0x1p-50;
-                v = v + fabs(v) * 0x1p-50;
-                if (Hc[e] == Hmax && o < 0.0) o = 0.0; // exact by construction of g
-            }
-            if (fp64) {
-                double2 *ov = (double2 *)a.ov + s.ov_off + i * s.n_eps + e;
-                *ov = make_double2(o, v);
-            } else {
-                float2 *ov = (float2 *)a.ov + s.ov_off + i * s.n_eps + e;
-                *ov = make_float2(f32_down(o), f32_up(v));
-            }
+            // Non-query rows, "none" and "all" eps: (-inf, -inf), nothing is counted here
+            // ("all" eps are counted in k_finalize).
+            a.lh[s.lh_off + i * s.n_eps + e] = query ? make_double2(Lc[e], Hc[e]) : make_double2(-INFINITY, -INFINITY);
         }
     }
     redh[threadIdx.x] = halfband;

@@ -23,9 +23,10 @@
 #include <math.h>
 
 // ---- classification of the join's candidate stream --------------------------------
-// One wavefront per chunk.  Per record and eps: certain-in (Z >= v) is counted,
-// inside the band (o <= Z < v) goes to the refine list, otherwise out.  Counts are
-// summed per distinct row of the wavefront before one atomic per row and eps.
+// One wavefront per chunk.  Per record, row and eps: certainly within eps is
+// counted, inside the band goes to the refine list, otherwise out.  A record of a
+// symmetric sweep decides for its row and for its candidate row.  Counts are summed
+// per distinct row of the wavefront before one atomic per row and eps.
 __device__ inline int set_of_row(const PassArgs &a, uint32_t row)
 {
     int lo = 0, hi = a.n_sets - 1;
@@ -37,13 +38,45 @@ __device__ inline int set_of_row(const PassArgs &a, uint32_t row)
     return lo;
 }
 
+// cnt[row][e] += 1 for every lane and every eps bit of `mask`; lanes with the same
+// global row are summed first (a vote loop over the distinct rows of the wavefront).
+// `li` is the dataset-local row of `key` in dataset `s`.  All lanes take part.
+__device__ inline void add_counts(const PassArgs &a, uint32_t key, const SetDesc &s, uint32_t li, uint32_t mask,
+                                  int lane)
+{
+    unsigned long long pending = __ballot(mask != 0u);
+    if (!pending) return;
+    uint32_t any = mask;
+    for (int o = 32; o > 0; o >>= 1) any |= (uint32_t)__shfl_xor((int)any, o);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t k = (uint32_t)__shfl((int)key, leader);
+        const bool same = key == k;
+        for (int e = 0; e < BFJ_MAXEPS; ++e) {
+            if (!((any >> e) & 1u)) continue;
+            const unsigned long long m = __ballot(same && ((mask >> e) & 1u));
+            if (m && lane == leader) atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + li], (uint32_t)__popcll(m));
+        }
+        pending &= ~__ballot(same);
+    }
+}
+
+// Decide the eps of one row from the bounds [lo, hi] of the pair's squared distance.
+__device__ inline void decide(const double2 *b, int n_eps, double lo, double hi, uint32_t &in, uint32_t &band)
+{
+    for (int e = 0; e < n_eps; ++e) {
+        const double2 p = b[e];
+        if (hi <= p.x) in |= 1u << e;
+        else if (lo <= p.y) band |= 1u << e;
+    }
+}
+
 __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t n_chunks = min(*a.chunk_count, a.chunk_capacity);
     const uint32_t wave_id = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
     const uint32_t n_waves = gridDim.x * (BFJ_BLOCK / BFJ_WAVE);
-    const bool fp64 = a.dtype == BFJ_FP64;
     for (uint32_t ch = wave_id; ch < n_chunks; ch += n_waves) {
         const uint32_t fill = min(a.chunk_fill[ch], (uint32_t)BFJ_CHUNK);
         for (uint32_t t0 = 0; t0 < fill; t0 += BFJ_WAVE) {
@@ -57,38 +90,23 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
             const int si = live ? (a.n_sets == 1 ? 0 : set_of_row(a, c.row)) : 0;
             const SetDesc &s = a.sets[si];
             const uint32_t i = c.row - (uint32_t)s.row_off;
-            uint32_t in_mask = 0, band = 0;
+            const uint32_t j = c.col & ~BFJ_COL_SYM;
+            uint32_t in_i = 0, band_i = 0, in_j = 0, band_j = 0;
             if (live) {
-                for (int e = 0; e < s.n_eps; ++e) {
-                    double o, v;
-                    if (fp64) {
-                        const double2 p = ((const double2 *)a.ov)[s.ov_off + (int64_t)i * s.n_eps + e];
-                        o = p.x;
-                        v = p.y;
-                    } else {
-                        const float2 p = ((const float2 *)a.ov)[s.ov_off + (int64_t)i * s.n_eps + e];
-                        o = p.x;
-                        v = p.y;
-                    }
-                    if (c.z >= v) in_mask |= 1u << e;
-                    else if (c.z >= o) band |= 1u << e;
-                }
+                // S_hat = 2 g_i + N_i - 2 Z, |S_hat - S| <= E_i; mg covers the FP64
+                // rounding of these few operations.
+                const double2 h = a.he[c.row];
+                const double shat = h.x - 2.0 * c.z;
+                const double mg = (fabs(h.x) + 2.0 * fabs(c.z) + h.y) * 0x1p-50;
+                const double lo = shat - h.y - mg, hi = shat + h.y + mg;
+                decide(a.lh + s.lh_off + (int64_t)i * s.n_eps, s.n_eps, lo, hi, in_i, band_i);
+                if (c.col & BFJ_COL_SYM) decide(a.lh + s.lh_off + (int64_t)j * s.n_eps, s.n_eps, lo, hi, in_j, band_j);
             }
-            // Certain-in counts, summed per distinct (row) of the wavefront.
-            unsigned long long pending = __ballot(in_mask != 0u);
-            while (pending) {
-                const int leader = __ffsll((long long)pending) - 1;
-                const uint32_t key = __shfl(c.row, leader);
-                const bool same = c.row == key && live;
-                for (int e = 0; e < BFJ_MAXEPS; ++e) {
-                    const unsigned long long m = __ballot(same && ((in_mask >> e) & 1u));
-                    if (m && lane == leader)
-                        atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + i], (uint32_t)__popcll(m));
-                }
-                pending &= ~__ballot(same);
-            }
+            add_counts(a, c.row, s, i, in_i, lane);
+            add_counts(a, (uint32_t)s.row_off + j, s, j, in_j, lane);
             // Band pairs to the refine list: one atomic per wavefront.
-            const unsigned long long bb = __ballot(band != 0u);
+            const bool band = (band_i | band_j) != 0u;
+            const unsigned long long bb = __ballot(band);
             if (bb) {
                 uint32_t base = 0;
                 if (lane == 0) base = atomicAdd(a.band_count, (uint32_t)__popcll(bb));
@@ -98,9 +116,9 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
                     if (idx < a.band_capacity) {
                         BandEntry be;
                         be.i = i;
-                        be.j = c.col;
+                        be.j = j;
                         be.z = (float)c.z;
-                        be.meta = band | ((uint32_t)si << 16);
+                        be.meta = band_i | (band_j << 8) | ((uint32_t)si << 16);
                         a.band[idx] = be;
                     }
                 }
@@ -174,34 +192,21 @@ __global__ __launch_bounds__(RWAVES * BFJ_WAVE) void k_refine(PassArgs a)
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         double err = -1.0;
-        // Hits per (dataset, row, eps): records of one group share few rows, so the
-        // lanes of each distinct (dataset, row) are summed first and one lane adds
-        // the sum (a vote loop over the distinct keys instead of contended atomics).
-        uint32_t hits = 0;
+        // Hits per (row, eps) for both rows of the record; lanes of the same row are
+        // summed first (records of one group share few rows).
+        uint32_t hits_i = 0, hits_j = 0;
         if (live) {
-            const uint32_t mask = be.meta & 0xffu;
             for (int e = 0; e < s.n_eps; ++e)
-                if (((mask >> e) & 1u) && acc <= s.T[e]) hits |= 1u << e;
+                if (acc <= s.T[e]) {
+                    hits_i |= be.meta & (1u << e);
+                    hits_j |= (be.meta >> 8) & (1u << e);
+                }
         }
-        const uint64_t key = live ? (((uint64_t)(uint32_t)si << 32) | be.i) : ~0ull;
-        unsigned long long pending = __ballot(hits != 0u);
-        while (pending) {
-            const int leader = __ffsll((long long)pending) - 1;
-            const uint64_t k = __shfl(key, leader);
-            const bool same = key == k;
-            const unsigned long long grp = __ballot(same && hits != 0u);
-            // Per-eps sums over the group (uniform loop: every lane takes part in the votes;
-            // the leader's own set and row address the counter).
-            for (int e = 0; e < BFJ_MAXEPS; ++e) {
-                const unsigned long long m = __ballot(same && ((hits >> e) & 1u));
-                if (m && lane == leader)
-                    atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + be.i], (uint32_t)__popcll(m));
-            }
-            pending &= ~grp;
-        }
+        add_counts(a, (uint32_t)s.row_off + be.i, s, be.i, hits_i, lane);
+        add_counts(a, (uint32_t)s.row_off + be.j, s, be.j, hits_j, lane);
         if (live) {
             const double sigma = a.scal[si].sigma;
-            const double approx = sigma * sigma * (a.hmax[s.row_off + be.i] - 2.0 * (double)be.z);
+            const double approx = sigma * sigma * (a.he[s.row_off + be.i].x - 2.0 * (double)be.z);
             err = fabs(approx - acc);
         }
         // Largest approximation error of the group: one atomic per dataset present.
