@@ -216,6 +216,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0, help="query rows timed on the CPU (0 = auto, ~10-30 s)")
+    ap.add_argument("--tile-rows", type=int, default=0, help="force the join's rows per wavefront (0 = library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (scales HBM byte counters)")
     args = ap.parse_args()
@@ -238,6 +239,8 @@ def main():
     else:
         n = args.n or {2: 100_000, 3: 1_000_000, 4: 500_000}[args.workload]
         dtype = args.dtype or {2: "fp64", 3: "fp16", 4: "bf16"}[args.workload]
+        if args.tile_rows:
+            ctx.set_tile_rows(dtype, args.tile_rows)
         log(f"generating workload {args.workload}: n={n}")
         x, eps = wl.config(args.workload, n=n)
         names = {2: ("pair-distances/s at n=100k,d=128 (FP64-MFMA + FP64 band refine, exact counts)", "uniform"),

@@ -46,7 +46,8 @@ typedef enum bfj_status {
     BFJ_ERR_DEVICE = 2,      /* HIP runtime error or bad device index */
     BFJ_ERR_NOMEM = 3,       /* device allocation failed */
     BFJ_ERR_UNSUPPORTED = 4, /* dimension or dtype outside what the kernels are built for */
-    BFJ_ERR_INTERNAL = 5
+    BFJ_ERR_INTERNAL = 5,
+    BFJ_ERR_CAPACITY = 6     /* output buffer too small; the call reports the size it needs */
 } bfj_status;
 
 /* Arithmetic of the dot products.  The band refine is FP64 in every case. */
@@ -113,6 +114,18 @@ int bfj_self_join_device(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t 
                          const double *eps, int32_t n_eps, int32_t dtype,
                          int64_t row_begin, int64_t row_end,
                          int64_t *d_counts, uint64_t *totals, bfj_stats *stats);
+
+/* The result set instead of counts: every unordered pair {i, j}, i < j, of the n
+ * rows with sqrt(d2(i, j)) <= eps (the reference's predicate, kernel_join.cu:194),
+ * written to pairs[2k], pairs[2k+1] (host, i < j) in no particular order.  The
+ * reference only counts pairs (kernel_join.cu:184, gpu_join.cu:321); its ordered
+ * self-inclusive total equals n + 2 * *n_pairs (eps >= 0).  *n_pairs is always set;
+ * when it exceeds capacity the call returns BFJ_ERR_CAPACITY and the caller
+ * retries with a buffer of that many pairs.  totals (may be NULL) as in
+ * bfj_self_join with n_eps = 1. */
+int bfj_self_join_pairs(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, int64_t ld,
+                        double eps, int32_t dtype, uint32_t *pairs, uint64_t capacity,
+                        uint64_t *n_pairs, uint64_t *total, bfj_stats *stats);
 
 /* Many independent datasets, each with its own eps list, in one persistent launch.
  * device_resident != 0: sets[i].data / counts are device pointers. */

@@ -29,7 +29,8 @@ MAX_EPS = 8
 MAX_DIM = 256
 
 BFJ_OK = 0
-STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "DEVICE", 3: "NOMEM", 4: "UNSUPPORTED", 5: "INTERNAL"}
+BFJ_ERR_CAPACITY = 6
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "DEVICE", 3: "NOMEM", 4: "UNSUPPORTED", 5: "INTERNAL", 6: "CAPACITY"}
 
 
 class JoinError(RuntimeError):
@@ -95,6 +96,8 @@ def load_library(path: str = LIB_PATH):
         "bfj_self_join": (ctypes.c_int, [vp, vp, i64, i32, i64, f64p, i32, i32, i64, i64, vp, u64p, sp]),
         "bfj_self_join_device": (ctypes.c_int, [vp, vp, i64, i32, i64, f64p, i32, i32, i64, i64, vp, u64p, sp]),
         "bfj_self_join_batch": (ctypes.c_int, [vp, ctypes.POINTER(Dataset), i32, i32, i32, sp]),
+        "bfj_self_join_pairs": (ctypes.c_int, [vp, vp, i64, i32, i64, ctypes.c_double, i32, vp, ctypes.c_uint64,
+                                               u64p, u64p, sp]),
         "bfj_plan_create": (ctypes.c_int, [vp, ctypes.POINTER(Dataset), i32, i32, ctypes.POINTER(vp)]),
         "bfj_plan_run": (ctypes.c_int, [vp, sp]),
         "bfj_plan_destroy": (ctypes.c_int, [vp]),
@@ -257,6 +260,31 @@ class Context:
             e.size, _dtype_id(dtype), row_begin, row_end, out.ctypes.data if counts and rows else None,
             tot.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(st)))
         return JoinResult(out, tot, st.as_dict())
+
+    def self_join_pairs(self, data, eps: float, dtype="fp16", capacity: int | None = None):
+        """Result set of the join: every unordered pair (i, j), i < j, with
+        sqrt(d2) <= eps, as a (m, 2) uint32 array in no particular order, plus the
+        reference's ordered self-inclusive total and the pass statistics.  The buffer
+        starts at ``capacity`` pairs (default 16 per row) and is grown once if the
+        library reports that it needs more."""
+        x = np.ascontiguousarray(data, dtype=np.float64)
+        if x.ndim != 2:
+            raise ValueError("data must be a 2-D float64 array")
+        n, d = x.shape
+        cap = int(capacity) if capacity is not None else max(16 * n, 1024)
+        for _ in range(2):
+            buf = np.zeros((max(cap, 1), 2), dtype=np.uint32)
+            found, tot = ctypes.c_uint64(0), ctypes.c_uint64(0)
+            st = Stats()
+            rc = self.lib.bfj_self_join_pairs(self._h, x.ctypes.data if n else None, n, d, d, float(eps),
+                                              _dtype_id(dtype), buf.ctypes.data, cap, ctypes.byref(found),
+                                              ctypes.byref(tot), ctypes.byref(st))
+            if rc == BFJ_ERR_CAPACITY and cap < found.value:
+                cap = int(found.value)
+                continue
+            _check(rc)
+            return buf[: found.value], int(tot.value), st.as_dict()
+        raise JoinError(BFJ_ERR_CAPACITY, "pair buffer still too small after growing it")
 
     def to_device(self, host, dtype=np.float64) -> DeviceArray:
         a = np.ascontiguousarray(host, dtype=dtype)

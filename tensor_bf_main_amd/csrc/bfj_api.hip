@@ -130,15 +130,18 @@ static int lds_for(int dtype, int kc, int rows, int n_eps)
     return 0;
 }
 
-// Default tile: the widest strip whose kernel still runs 2 blocks (8 waves) per
-// CU, else the widest that was built.
+// Default tile: the most resident blocks per CU (counted up to 3: latency hiding
+// across blocks matters more than strip width), then the widest strip.
 static int heuristic_rows(int dtype, int kc, int n_eps)
 {
-    int best = 0;
+    int best = 0, best_occ = -1;
     for (int r : kRowOptions)
         if (bfj_join_has(dtype, kc, r)) {
-            const int occ = bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps));
-            if (occ >= 2 || best == 0) best = r;
+            const int occ = std::min(3, bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps)));
+            if (occ >= best_occ) {
+                best = r;
+                best_occ = occ;
+            }
         }
     return best;
 }
@@ -173,9 +176,11 @@ struct bfj_ctx {
     int n_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
-    DevBuf pack, nq, gq, lh, he, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
-        outbuf, cand, chunk_fill;
+    DevBuf pack, nq, gq, he, row_set, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
+        outbuf, cand, chunk_fill, pairs;
     uint32_t band_capacity = 0;
+    uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
+    int emit_pairs = 0;
     uint32_t chunk_capacity = 0; // candidate chunks of BFJ_CHUNK records
     int tile_rows[4][33] = {}; // [dtype][kc] forced/tuned R (0 = heuristic)
 };
@@ -231,7 +236,7 @@ struct PassPlan {
     std::vector<Group> groups;
     std::vector<int32_t> blk_set, set_blk0, set_nblk;
     std::vector<int64_t> blk_row0;
-    int64_t rows_total = 0, pack_bytes = 0, lh_total = 0, cnt_total = 0;
+    int64_t rows_total = 0, pack_bytes = 0, cnt_total = 0;
     uint64_t pairs = 0;
     int max_rows = 0, max_kc = 0;
 };
@@ -276,15 +281,14 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         s.n_eps = h.n_eps;
         s.set_id = (int32_t)si;
         s.sym = (h.row_begin == 0 && h.row_end == h.n) ? 1 : 0;
+        s.al16 = ((uintptr_t)h.x % 16 == 0 && h.ld % 2 == 0) ? 1 : 0;
         s.out = h.out;
         for (int e = 0; e < h.n_eps; ++e) s.T[e] = eps_threshold(h.eps[e]);
         s.row_off = p.rows_total;
         s.pack_off = p.pack_bytes;
-        s.lh_off = p.lh_total;
         s.cnt_off = p.cnt_total;
         p.rows_total += s.npad;
         p.pack_bytes += (s.npad / 16) * (int64_t)s.kc * BFJ_WAVE * FB;
-        p.lh_total += s.npad * s.n_eps;
         p.cnt_total += s.npad * s.n_eps;
         p.pairs += (uint64_t)(h.row_end - h.row_begin) * (uint64_t)h.n;
         p.max_kc = std::max(p.max_kc, s.kc);
@@ -349,8 +353,8 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->pack, std::max<int64_t>(p.pack_bytes, 64)));
     RC(ensure(ctx->nq, nrows * vb));
     RC(ensure(ctx->gq, nrows * vb));
-    RC(ensure(ctx->lh, std::max<int64_t>(p.lh_total, 1) * 16));
     RC(ensure(ctx->he, nrows * 16));
+    RC(ensure(ctx->row_set, nrows * 2));
     RC(ensure(ctx->qerr, nrows * 8));
     RC(ensure(ctx->nrm, nrows * 8));
     RC(ensure(ctx->scales, nrows));
@@ -409,8 +413,8 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.pack = (unsigned char *)ctx->pack.p;
     a.nq = ctx->nq.p;
     a.gq = ctx->gq.p;
-    a.lh = (double2 *)ctx->lh.p;
     a.he = (double2 *)ctx->he.p;
+    a.row_set = (uint16_t *)ctx->row_set.p;
     a.qerr = (double *)ctx->qerr.p;
     a.nrm = (double *)ctx->nrm.p;
     a.scales = (uint8_t *)ctx->scales.p;
@@ -423,6 +427,10 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.chunk_fill = (const uint32_t *)ctx->chunk_fill.p;
     a.chunk_count = (const uint32_t *)ctx->counters.p + 1;
     a.chunk_capacity = ctx->chunk_capacity;
+    a.emit_pairs = ctx->emit_pairs;
+    a.pairs = (uint2 *)ctx->pairs.p;
+    a.pair_count = (unsigned long long *)((uint32_t *)ctx->counters.p + 4);
+    a.pair_capacity = ctx->pair_capacity;
     return a;
 }
 
@@ -636,9 +644,9 @@ int bfj_destroy(bfj_ctx *c)
     if (!c) return BFJ_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->lh, &c->he, &c->qerr, &c->nrm, &c->scales, &c->cnt,
+    DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
                       &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
-                      &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill};
+                      &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -707,6 +715,43 @@ int bfj_self_join(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, int64_
                 counts ? (int64_t *)ctx->outbuf.p : nullptr, totals, stats, 0));
     if (counts && row_end > row_begin)
         HC(hipMemcpy(counts, ctx->outbuf.p, (size_t)(row_end - row_begin) * n_eps * 8, hipMemcpyDeviceToHost));
+    return BFJ_OK;
+}
+
+int bfj_self_join_pairs(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, int64_t ld, double eps, int32_t dtype,
+                        uint32_t *pairs, uint64_t capacity, uint64_t *n_pairs, uint64_t *total, bfj_stats *stats)
+{
+    RC(common_checks(ctx, data, n, &eps, 1));
+    if (!n_pairs) return fail(BFJ_ERR_INVALID, "n_pairs is NULL");
+    *n_pairs = 0;
+    if (capacity > 0 && !pairs) return fail(BFJ_ERR_INVALID, "pairs is NULL");
+    if (d < 1 || ld < d) return fail(BFJ_ERR_INVALID, "need 1 <= d <= ld");
+    if (n > 0) {
+        RC(ensure(ctx->xcopy, (size_t)n * (size_t)ld * 8));
+        HC(hipMemcpyAsync(ctx->xcopy.p, data, (size_t)n * ld * 8, hipMemcpyHostToDevice, ctx->stream));
+    }
+    RC(ensure(ctx->pairs, (size_t)std::max<uint64_t>(capacity, 1) * 8));
+    ctx->pair_capacity = capacity;
+    ctx->emit_pairs = 1;
+    uint64_t tot = 0;
+    const int rc = join_one(ctx, (const double *)ctx->xcopy.p, n, d, ld, &eps, 1, dtype, 0, n, nullptr, &tot, stats, 0);
+    ctx->emit_pairs = 0;
+    ctx->pair_capacity = 0;
+    RC(rc);
+    unsigned long long found = 0;
+    SetScalars sc;
+    HC(hipMemcpy(&found, (uint32_t *)ctx->counters.p + 4, 8, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(&sc, ctx->scal.p, sizeof sc, hipMemcpyDeviceToHost));
+    const bool all = (sc.all_mask & 1ull) != 0; // eps beyond every distance: the join left them to us
+    if (all) found = n > 1 ? (unsigned long long)n * (unsigned long long)(n - 1) / 2 : 0;
+    *n_pairs = found;
+    if (total) *total = tot;
+    if (found > capacity)
+        return fail(BFJ_ERR_CAPACITY, "pairs buffer holds %llu pairs, %llu needed", (unsigned long long)capacity, found);
+    if (all) bfj_launch_all_pairs((uint2 *)ctx->pairs.p, n, ctx->stream);
+    HC(hipGetLastError());
+    HC(hipStreamSynchronize(ctx->stream));
+    if (found) HC(hipMemcpy(pairs, ctx->pairs.p, found * 8, hipMemcpyDeviceToHost));
     return BFJ_OK;
 }
 

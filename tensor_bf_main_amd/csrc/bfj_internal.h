@@ -9,8 +9,6 @@
 //   nq[row]    0.5 * ||xhat_j||^2 (FP32 or FP64), +BIG for padding rows
 //   gq[row]    g_i = 0.5 * (H* + E_i - N_i) rounded up (H*: largest banded H of the
 //              dataset); -BIG for rows that are not queries
-//   lh[row][e] (L, H): squared-distance bounds (kernel space) of row i for each
-//              epsilon: S <= L is certainly within eps, S > H certainly not
 //   he[row]    (2 g_i + N_i, E_i): S_hat = he.x - 2 Z estimates S with |S_hat - S| <= E_i
 //   cnt[e][row] per-row hit counts, self included (uint32)
 //   band list  (i, j, z, mask) records of pairs whose band straddles a threshold
@@ -42,7 +40,6 @@ struct SetDesc {
     int64_t row_end;
     int64_t row_off;   // offset in the row-indexed arrays
     int64_t pack_off;  // byte offset of this dataset's fragments in pack
-    int64_t lh_off;    // offset (in bound pairs) into lh
     int64_t cnt_off;   // offset into cnt; cnt[cnt_off + e * npad + i]
     int64_t *out;      // device int64 counts [n_eps][row_end-row_begin], self excluded (may be NULL)
     int32_t d;         // true dimension
@@ -50,6 +47,7 @@ struct SetDesc {
     int32_t n_eps;
     int32_t set_id;
     int32_t sym;       // every row is a query: each unordered pair is evaluated once
+    int32_t al16;      // x and ld keep every row 16-byte aligned
     double T[BFJ_MAXEPS]; // input-space thresholds: hit iff d2 <= T
 };
 
@@ -63,6 +61,9 @@ struct SetScalars {
     unsigned long long halfband_bits; // max (H - L) / 2 (input space)
     unsigned long long nonfinite;     // non-zero if some row holds NaN or inf
     unsigned long long all_mask;      // eps whose T exceeds every possible distance: all pairs hit
+    unsigned long long band_mask;     // eps decided by the join + refine (neither none nor all)
+    double slo[BFJ_MAXEPS];           // sqrt(T/(1+rho)) rounded down (kernel space)
+    double shi[BFJ_MAXEPS];           // sqrt(T/(1-rho)) rounded up
     unsigned long long total[BFJ_MAXEPS]; // totals, self included
 };
 
@@ -139,8 +140,8 @@ struct PassArgs {
     double *mu;                // [n_sets][BFJ_MAXDIM]
     unsigned char *pack;
     void *nq, *gq;
-    double2 *lh;               // [lh_off + i * n_eps + e], i dataset-local
     double2 *he;               // per row
+    uint16_t *row_set;         // per row: index of its dataset
     double *qerr, *nrm;        // per row
     uint8_t *scales;
     uint32_t *cnt;
@@ -152,6 +153,10 @@ struct PassArgs {
     const uint32_t *chunk_fill;
     const uint32_t *chunk_count;
     uint32_t chunk_capacity;
+    int32_t emit_pairs;                // pair-list mode: one dataset, eps 0, every row a query
+    uint2 *pairs;                      // (i, j), i < j
+    unsigned long long *pair_count;    // pairs found (may exceed pair_capacity)
+    unsigned long long pair_capacity;
 };
 
 #define BFJ_MAXDIM 256
@@ -162,3 +167,4 @@ void bfj_launch_prep(const PassArgs &a, hipStream_t s);
 void bfj_launch_classify(const PassArgs &a, hipStream_t s);
 void bfj_launch_refine(const PassArgs &a, hipStream_t s);
 void bfj_launch_finalize(const PassArgs &a, hipStream_t s);
+void bfj_launch_all_pairs(uint2 *pairs, int64_t n, hipStream_t s);

@@ -13,11 +13,11 @@
 //     load per block instead of one per wavefront (the reference reads
 //     candidates from global memory in every warp, kernel_join.cu:432), no
 //     VGPRs spent on staging, one barrier per candidate block.
-//   * The accumulator of each 16x16 tile starts at C0 = g_i - n_j, so after
-//     the MFMAs Z = 0.5*(H_i - d2_approx(i, j)) with no further arithmetic: the
-//     squared norms and the threshold are folded into the initial value
-//     (the reference pre-loads candidate chunk norms the same way,
-//     kernel_join.cu:433-437).
+//   * The first MFMA of each row block takes g_i as its C operand, so after the
+//     MFMAs acc = g_i + xhat_i . xhat_j and Z = acc - n_j = 0.5*(2 g_i + N_i -
+//     d2_approx(i, j)): the squared norms and the threshold are folded in with
+//     no VALU work before the MFMAs, and the test Z >= 0 is acc >= n_j (the
+//     reference pre-loads candidate chunk norms, kernel_join.cu:433-437).
 //   * Z < 0 means "certainly out" for every epsilon.  One max tree and one wave
 //     vote skip almost every tile.  The pairs of a hot tile with Z >= 0 are
 //     appended to a candidate stream (row, col, Z) with wave-compacted plain
@@ -100,11 +100,17 @@ struct JoinLds {
     static constexpr int FRAG = UPW * BFJ_BLOCK * 16;
     static constexpr int NQB = 16 * (int)sizeof(val_t); // candidate norms of the tile
     static constexpr int BUF = FRAG + NQB + 16;          // + FP8 scales
-    static constexpr int NB = BUF <= 4608 ? 4 : (BUF <= 9216 ? 3 : 2); // ring depth
+    // Ring depth: about 16 KiB of tiles per block, at least double buffering.  A
+    // deeper ring buys little; the LDS it would take is worth more as occupancy.
+    static constexpr int NB_RAW = (16 * 1024) / BUF;
+    static constexpr int NB = NB_RAW < 2 ? 2 : (NB_RAW > 16 ? 16 : NB_RAW);
     static constexpr int LPT = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0); // DMA instructions per tile
     static constexpr int RING = NB * BUF;
     static constexpr int RB = R / 16;
-    __host__ __device__ static constexpr int total(int) { return RING; }
+    // Per-wavefront staging of a hot tile's accumulators (lane-private, so the
+    // candidate push can loop over row blocks without holding them in VGPRs).
+    static constexpr int STAGE = RB * BFJ_WAVE * (int)sizeof(typename Tr::acc_t);
+    __host__ __device__ static constexpr int total(int) { return RING + BFJ_JWAVES * STAGE; }
 };
 
 // Per-wavefront cursor into the candidate stream: chunks of BFJ_CHUNK records
@@ -120,26 +126,49 @@ __device__ inline void cand_close(const JoinArgs &a, const CandCursor &cc, int l
     if (cc.chunk != 0xffffffffu && cc.chunk < a.chunk_capacity && lane == 0) a.chunk_fill[cc.chunk] = cc.fill;
 }
 
-// Append the lanes with `take` set as candidate records (wave-compacted).
-__device__ inline void cand_push(const JoinArgs &a, CandCursor &cc, bool take, uint32_t row, uint32_t col, double z,
-                                 int lane)
+// Append the pairs of one 16x16 output block with acc >= n_j (Z >= 0) as candidate
+// records, wave-compacted: one ballot per register, at most one chunk grab, then
+// plain 16-byte stores.  In a symmetric sweep only pairs above the diagonal
+// (col > row) are kept: the tile that holds (j, i) is dropped by the caller and
+// self pairs are counted in k_finalize.
+template <class Tr>
+__device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &cc, const typename Tr::acc_t &v,
+                                                typename Tr::val_t nj, int col, bool col_ok, int li0, bool sym,
+                                                uint32_t row0, int lane)
 {
-    const unsigned long long bal = __ballot(take);
-    if (!bal) return;
-    const uint32_t nrec = (uint32_t)__popcll(bal);
-    if (cc.fill + nrec > BFJ_CHUNK) {
+    unsigned long long bal[4];
+    uint32_t cnt[4], nrec = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int li = li0 + Tr::out_row(lane, r);
+        bal[r] = __ballot(v[r] >= nj && col_ok && (!sym || col > li));
+        cnt[r] = (uint32_t)__popcll(bal[r]);
+        nrec += cnt[r];
+    }
+    if (nrec == 0) return;
+    if (cc.fill + nrec > BFJ_CHUNK) { // nrec <= 64 fits a fresh chunk
         cand_close(a, cc, lane);
         uint32_t c = 0;
         if (lane == 0) c = atomicAdd(a.chunk_count, 1u);
         cc.chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(c, 0));
         cc.fill = 0;
     }
-    if (take && cc.chunk < a.chunk_capacity) {
-        Candidate rec;
-        rec.row = row;
-        rec.col = col;
-        rec.z = z;
-        a.cand[(size_t)cc.chunk * BFJ_CHUNK + cc.fill + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = rec;
+    if (cc.chunk < a.chunk_capacity) {
+        const unsigned long long below = (1ull << lane) - 1ull;
+        Candidate *dst = a.cand + (size_t)cc.chunk * BFJ_CHUNK + cc.fill;
+        const uint32_t tag = sym ? BFJ_COL_SYM : 0u;
+        uint32_t off = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if ((bal[r] >> lane) & 1ull) {
+                Candidate rec;
+                rec.row = row0 + (uint32_t)Tr::out_row(lane, r);
+                rec.col = (uint32_t)col | tag;
+                rec.z = (double)(v[r] - nj);
+                dst[off + (uint32_t)__popcll(bal[r] & below)] = rec;
+            }
+            off += cnt[r];
+        }
     }
     cc.fill += nrec;
 }
@@ -154,9 +183,9 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     typedef JoinLds<Tr, KC, R> L;
     constexpr int RB = R / 16;
     const int lane = bfj_lane();
-    const int wave = threadIdx.x / BFJ_WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BFJ_WAVE));
     const int64_t nblk = s.npad / 16;
-    const int64_t wrb0 = (int64_t)rb0 + wave * RB; // first row block of this wavefront
+    const int wrb0 = rb0 + wave * RB; // first row block of this wavefront
     const frag_t *P = reinterpret_cast<const frag_t *>(a.pack + s.pack_off);
     const val_t *nq = reinterpret_cast<const val_t *>(a.nq) + s.row_off;
     const val_t *gq = reinterpret_cast<const val_t *>(a.gq) + s.row_off;
@@ -164,7 +193,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 
     // Fragments and initial values of this wavefront's rows, resident in VGPRs.
     frag_t A[RB][KC];
-    val_t G[RB][4];
+    acc_t G[RB]; // g_i per output register: the C operand of each row block's first MFMA
     int SA[RB];
     bool any_rows = false;
 #pragma unroll
@@ -190,21 +219,27 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     typedef __attribute__((address_space(3))) void lds_void;
     const unsigned char *srcb = reinterpret_cast<const unsigned char *>(P);
     const unsigned char *nqb = reinterpret_cast<const unsigned char *>(nq);
-    auto issue = [&](int64_t t, int bi) {
-        unsigned char *b = lds + bi * L::BUF;
+    constexpr int NQW = L::NQB / BFJ_JWAVES; // norm bytes per wavefront
+    uint32_t voff[L::UPW];                   // this lane's byte offset in a tile, per load
 #pragma unroll
-        for (int u = 0; u < L::UPW; ++u) {
-            const int unit = u * BFJ_BLOCK + wave * BFJ_WAVE + lane;
-            const unsigned char *g = srcb + (t * L::UNITS + (unit < L::UNITS ? unit : 0)) * 16;
-            __builtin_amdgcn_global_load_lds(g, (lds_void *)(b + (u * BFJ_BLOCK + wave * BFJ_WAVE) * 16), 16, 0, 0);
-        }
-        constexpr int NQW = L::NQB / BFJ_JWAVES; // norm bytes per wavefront
+    for (int u = 0; u < L::UPW; ++u) {
+        const int unit = u * BFJ_BLOCK + wave * BFJ_WAVE + lane;
+        voff[u] = (uint32_t)(unit < L::UNITS ? unit : 0) * 16u;
+    }
+    // t and slot are uniform, so the addresses are a scalar base plus a lane offset.
+    auto issue = [&](int t, int slot) {
+        unsigned char *b = lds + slot * L::BUF;
+        const unsigned char *gt = srcb + (size_t)t * L::TILE;
+#pragma unroll
+        for (int u = 0; u < L::UPW; ++u)
+            __builtin_amdgcn_global_load_lds(gt + voff[u], (lds_void *)(b + (u * BFJ_BLOCK + wave * BFJ_WAVE) * 16), 16,
+                                             0, 0);
         if (lane < NQW / 4)
-            __builtin_amdgcn_global_load_lds(nqb + t * L::NQB + wave * NQW + lane * 4,
+            __builtin_amdgcn_global_load_lds(nqb + (size_t)t * L::NQB + wave * NQW + lane * 4,
                                              (lds_void *)(b + L::FRAG + wave * NQW), 4, 0, 0);
         if (Tr::dtype == BFJ_FP8 && lane == 0)
-            __builtin_amdgcn_global_load_lds(sc + t * 16 + wave * 4, (lds_void *)(b + L::FRAG + L::NQB + wave * 4), 4,
-                                             0, 0);
+            __builtin_amdgcn_global_load_lds(sc + (size_t)t * 16 + wave * 4, (lds_void *)(b + L::FRAG + L::NQB + wave * 4),
+                                             4, 0, 0);
     };
     // vmcnt <= (NB-2)*LPT: of this wavefront's loads only the NB-2 newest tiles
     // may still be in flight, so the oldest one (the tile about to be read) landed.
@@ -219,13 +254,13 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     // A dataset whose rows are all queries sweeps only candidate blocks from the
     // strip's first block on: every unordered pair lies in exactly one tile (rb, cb)
     // with cb >= rb, and that tile decides it for both rows.
-    const int64_t cb0 = s.sym ? (int64_t)rb0 : 0;
+    const int nb = (int)nblk; // n < 2^31, so block indices fit in 32 bits
+    const int cb0 = s.sym ? rb0 : 0;
+    // Tile cb lives in ring slot (cb - cb0) mod NB.
 #pragma unroll
-    for (int t = 0; t < L::NB - 1; ++t) {
-        const int64_t tt = cb0 + t;
-        issue(tt < nblk ? tt : nblk - 1, (int)(tt % L::NB));
-    }
-    for (int64_t cb = cb0; cb < nblk; ++cb) {
+    for (int t = 0; t < L::NB - 1; ++t) issue(min(cb0 + t, nb - 1), t);
+    int rs = 0; // slot of tile cb
+    for (int cb = cb0; cb < nb; ++cb) {
         __builtin_amdgcn_s_waitcnt(WAIT_IMM);
         // All quarters of tile cb landed (each wavefront waited for its own loads
         // above); every wavefront is done with tile cb-1, whose fragments fed its
@@ -233,59 +268,53 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         // the direct-to-LDS loads, and a release would drain the ones in flight.
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        {
-            const int64_t t = cb + L::NB - 1; // re-uses the buffer of tile cb-1
-            issue(t < nblk ? t : nblk - 1, (int)(t % L::NB));
-        }
+        issue(min(cb + L::NB - 1, nb - 1), rs == 0 ? L::NB - 1 : rs - 1); // into the slot of tile cb-1
+        const int rcur = rs;
+        rs = rs + 1 == L::NB ? 0 : rs + 1;
         if (any_rows) {
-            const unsigned char *b = lds + (int)(cb % L::NB) * L::BUF;
+            const unsigned char *b = lds + rcur * L::BUF;
             frag_t B[KC];
 #pragma unroll
             for (int c = 0; c < KC; ++c) B[c] = reinterpret_cast<const frag_t *>(b)[c * BFJ_WAVE + lane];
             const val_t nj = reinterpret_cast<const val_t *>(b + L::FRAG)[lane & 15];
             const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + (lane & 15)] : 127;
 
+            // acc = g_i + xhat_i . xhat_j; the first MFMA reads g_i as its C operand, so
+            // no VALU work precedes the MFMAs.  Z = acc - n_j is only formed for hot
+            // tiles: acc < n_j is Z < 0 exactly, and max commutes with subtracting the
+            // per-lane n_j.
             acc_t acc[RB];
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
+            for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][0], B[0], G[rb], SA[rb], SB);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[rb][r] = G[rb][r] - nj;
-#pragma unroll
-            for (int c = 0; c < KC; ++c)
+            for (int c = 1; c < KC; ++c)
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][c], B[c], acc[rb], SA[rb], SB);
 
             // Inputs are NaN-free (the pass rejects non-finite rows), so with
-            // -fno-honor-nans this is a tree of v_max3.  -0.0 >= 0 holds, so a zero Z
-            // of either sign is kept.
+            // -fno-honor-nans this is a tree of v_max3.
             val_t m = acc[0][0];
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) m = fmax(m, acc[rb][r]);
 
-            if (__builtin_expect(__ballot(m >= (val_t)0) != 0ull, 0)) {
-                // Rare: stream every pair with Z >= 0 as a candidate record (wave
-                // compacted, plain stores); k_classify applies the per-eps tests.
-                // In a symmetric sweep the tiles with cb < rb and the lower half of
-                // cb == rb belong to another tile and are dropped here.
-                const int64_t col = cb * 16 + (lane & 15);
+            if (__builtin_expect(__ballot(m >= nj) != 0ull, 0)) {
+                // Rare: stream every pair with acc >= n_j (Z >= 0) as a candidate
+                // record; k_classify applies the per-eps tests to Z = acc - n_j.  In a
+                // symmetric sweep the tiles with cb < rb belong to another tile.
+                const int col = cb * 16 + (lane & 15);
                 const bool col_ok = col < s.n;
+                const bool sym = s.sym != 0;
+                acc_t *stage = reinterpret_cast<acc_t *>(lds + L::RING + wave * L::STAGE);
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) {
-                    const int64_t blk = wrb0 + rb;
-                    if (s.sym && cb < blk) continue;
-                    const val_t mr = fmax(fmax(acc[rb][0], acc[rb][1]), fmax(acc[rb][2], acc[rb][3]));
-                    if (__ballot(mr >= (val_t)0 && col_ok) == 0ull) continue;
+                for (int rb = 0; rb < RB; ++rb) stage[rb * BFJ_WAVE + lane] = acc[rb];
 #pragma unroll 1
-                    for (int r = 0; r < 4; ++r) {
-                        const val_t z = r == 0 ? acc[rb][0] : (r == 1 ? acc[rb][1] : (r == 2 ? acc[rb][2] : acc[rb][3]));
-                        const int64_t li = blk * 16 + Tr::out_row(lane, r);
-                        const bool keep = col_ok && (!s.sym || col >= li);
-                        const uint32_t tag = (s.sym && col != li) ? BFJ_COL_SYM : 0u;
-                        cand_push(a, cc, z >= (val_t)0 && keep, (uint32_t)(s.row_off + li), (uint32_t)col | tag,
-                                  (double)z, lane);
-                    }
+                for (int rb = 0; rb < RB; ++rb) {
+                    const int blk = wrb0 + rb;
+                    if (sym && cb < blk) continue;
+                    cand_push_block<Tr>(a, cc, stage[rb * BFJ_WAVE + lane], nj, col, col_ok, blk * 16, sym,
+                                        (uint32_t)(s.row_off + blk * 16), lane);
                 }
             }
         }

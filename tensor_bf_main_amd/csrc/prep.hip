@@ -84,17 +84,25 @@ __global__ void k_colsum(PassArgs a)
     }
 }
 
+// One block per (dataset, column): the block sums the column's partial sums in a
+// fixed order (strided per thread, then a tree), so the mean is reproducible.
 __global__ void k_colmean(PassArgs a)
 {
-    const int32_t *set_blk0 = a.set_blk0, *set_nblk = a.set_nblk;
-    const int si = blockIdx.x;
+    __shared__ double red[BFJ_BLOCK];
+    const int si = blockIdx.x, k = blockIdx.y;
     const SetDesc &s = a.sets[si];
-    for (int k = threadIdx.x; k < BFJ_MAXDIM; k += blockDim.x) {
-        double acc = 0.0;
-        if (k < s.d)
-            for (int b = 0; b < set_nblk[si]; ++b) acc += a.colsum[(int64_t)(set_blk0[si] + b) * BFJ_MAXDIM + k];
-        a.mu[(int64_t)si * BFJ_MAXDIM + k] = (s.n > 0 && k < s.d) ? acc / (double)s.n : 0.0;
+    double acc = 0.0;
+    if (k < s.d) {
+        const int64_t b0 = a.set_blk0[si];
+        for (int b = threadIdx.x; b < a.set_nblk[si]; b += BFJ_BLOCK) acc += a.colsum[(b0 + b) * BFJ_MAXDIM + k];
     }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = BFJ_BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.mu[(int64_t)si * BFJ_MAXDIM + k] = (s.n > 0 && k < s.d) ? red[0] / (double)s.n : 0.0;
 }
 
 // ---- scale ------------------------------------------------------------------
@@ -383,10 +391,19 @@ __global__ void k_thresholds(PassArgs a)
         if (fp64) ((double *)a.gq)[row] = g;
         else ((float *)a.gq)[row] = (float)g;
         a.he[row] = make_double2(2.0 * g + Ni, E);
-        for (int e = 0; e < s.n_eps; ++e) {
-            // Non-query rows, "none" and "all" eps: (-inf, -inf), nothing is counted here
-            // ("all" eps are counted in k_finalize).
-            a.lh[s.lh_off + i * s.n_eps + e] = query ? make_double2(Lc[e], Hc[e]) : make_double2(-INFINITY, -INFINITY);
+        a.row_set[row] = (uint16_t)si;
+        if (i == 0) {
+            // Per-dataset radii of the banded eps; k_classify forms the bounds of each
+            // pair from them and the pair's own r = q_i + q_j.
+            uint32_t band_mask = 0;
+            for (int e = 0; e < s.n_eps; ++e) {
+                const double T = s.T[e] / sigma2;
+                const bool banded = T >= 0.0 && !((all_mask >> e) & 1u);
+                if (banded) band_mask |= 1u << e;
+                a.scal[si].slo[e] = banded ? sqrt(T / (1.0 + rho)) * (1.0 - 0x1p-50) : 0.0;
+                a.scal[si].shi[e] = banded ? sqrt(T / (1.0 - rho)) * (1.0 + 0x1p-50) : 0.0;
+            }
+            a.scal[si].band_mask = band_mask;
         }
     }
     redh[threadIdx.x] = halfband;
@@ -401,7 +418,7 @@ __global__ void k_thresholds(PassArgs a)
 void bfj_launch_prep(const PassArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(k_colsum, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
-    hipLaunchKernelGGL(k_colmean, dim3(a.n_sets), dim3(BFJ_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_colmean, dim3(a.n_sets, BFJ_MAXDIM), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_maxabs, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_sigma, dim3((a.n_sets + 255) / 256), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_quantize, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);

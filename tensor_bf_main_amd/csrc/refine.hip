@@ -12,13 +12,11 @@
 // the predicate sqrt(acc) <= eps of kernel_join.cu:194.  These are the only
 // FP64 steps of the low-precision paths besides the threshold computation.
 //
-// Layout of the work: one wavefront takes 64 band records, one per lane.  The
-// two rows of every record are streamed through LDS in chunks of RCH doubles:
-// the wavefront loads the 128 row segments of a chunk with coalesced 8-byte
-// loads (16 lanes per 128-byte segment) and stores them transposed,
-// lds[k][pair], so that each lane then runs its own sequential FMA chain with
-// conflict-free LDS reads.  One HBM/L2 line per row segment instead of one per
-// lane per word.
+// Layout of the work: one lane per band record.  Each lane streams the two rows
+// of its pair through registers with 16-byte loads, RK doubles at a time, the
+// next chunk in flight while the current one feeds the FMA chain; the many
+// wavefronts per SIMD hide the rest of the latency.  Rows of the pairs of one
+// wavefront are shared in L2 (a strip's records cluster on few rows).
 #include "bfj_internal.h"
 #include <math.h>
 
@@ -27,17 +25,6 @@
 // counted, inside the band goes to the refine list, otherwise out.  A record of a
 // symmetric sweep decides for its row and for its candidate row.  Counts are summed
 // per distinct row of the wavefront before one atomic per row and eps.
-__device__ inline int set_of_row(const PassArgs &a, uint32_t row)
-{
-    int lo = 0, hi = a.n_sets - 1;
-    while (lo < hi) { // last set with row_off <= row
-        const int mid = (lo + hi + 1) >> 1;
-        if ((uint64_t)a.sets[mid].row_off <= row) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 // cnt[row][e] += 1 for every lane and every eps bit of `mask`; lanes with the same
 // global row are summed first (a vote loop over the distinct rows of the wavefront).
 // `li` is the dataset-local row of `key` in dataset `s`.  All lanes take part.
@@ -61,13 +48,35 @@ __device__ inline void add_counts(const PassArgs &a, uint32_t key, const SetDesc
     }
 }
 
-// Decide the eps of one row from the bounds [lo, hi] of the pair's squared distance.
-__device__ inline void decide(const double2 *b, int n_eps, double lo, double hi, uint32_t &in, uint32_t &band)
+// Pair-list mode: append (i, j) for the lanes with `take` (one atomic per wavefront).
+// Pairs past the capacity are counted, not written.  All lanes take part.
+__device__ inline void append_pairs(const PassArgs &a, bool take, uint32_t i, uint32_t j, int lane)
+{
+    const unsigned long long bal = __ballot(take);
+    if (!bal) return;
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(a.pair_count, (unsigned long long)__popcll(bal));
+    base = __shfl(base, 0);
+    if (take) {
+        const unsigned long long idx = base + (unsigned long long)__popcll(bal & ((1ull << lane) - 1ull));
+        if (idx < a.pair_capacity) a.pairs[idx] = make_uint2(i, j);
+    }
+}
+
+// Decide every banded eps of a pair from the bounds [lo, hi] of its squared
+// distance S (kernel space) and its quantisation radius r = q_i + q_j:
+//   certainly within eps if hi <= L = (slo - r)^2, certainly outside if lo > H = (shi + r)^2
+// (rounded outwards), in the band otherwise.
+__device__ inline void decide(const SetScalars &sc, uint32_t band_mask, int n_eps, double r, double lo, double hi,
+                              uint32_t &in, uint32_t &band)
 {
     for (int e = 0; e < n_eps; ++e) {
-        const double2 p = b[e];
-        if (hi <= p.x) in |= 1u << e;
-        else if (lo <= p.y) band |= 1u << e;
+        if (!((band_mask >> e) & 1u)) continue;
+        const double slo = sc.slo[e], shi = sc.shi[e];
+        const double L = slo > r ? (slo - r) * (slo - r) * (1.0 - 0x1p-49) : -INFINITY;
+        const double H = (shi + r) * (shi + r) * (1.0 + 0x1p-49);
+        if (hi <= L) in |= 1u << e;
+        else if (lo <= H) band |= 1u << e;
     }
 }
 
@@ -87,7 +96,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
             c.col = 0;
             c.z = -1.0;
             if (live) c = a.cand[(size_t)ch * BFJ_CHUNK + t];
-            const int si = live ? (a.n_sets == 1 ? 0 : set_of_row(a, c.row)) : 0;
+            const int si = live ? (int)a.row_set[c.row] : 0;
             const SetDesc &s = a.sets[si];
             const uint32_t i = c.row - (uint32_t)s.row_off;
             const uint32_t j = c.col & ~BFJ_COL_SYM;
@@ -99,11 +108,19 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
                 const double shat = h.x - 2.0 * c.z;
                 const double mg = (fabs(h.x) + 2.0 * fabs(c.z) + h.y) * 0x1p-50;
                 const double lo = shat - h.y - mg, hi = shat + h.y + mg;
-                decide(a.lh + s.lh_off + (int64_t)i * s.n_eps, s.n_eps, lo, hi, in_i, band_i);
-                if (c.col & BFJ_COL_SYM) decide(a.lh + s.lh_off + (int64_t)j * s.n_eps, s.n_eps, lo, hi, in_j, band_j);
+                // |dist(xhat_i, xhat_j) - dist(y_i, y_j)| <= q_i + q_j: the same bounds
+                // hold for both rows of the pair.
+                const double r = (a.qerr[c.row] + a.qerr[s.row_off + j]) * (1.0 + 0x1p-50);
+                const SetScalars &sc = a.scal[si];
+                decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, lo, hi, in_i, band_i);
+                if (c.col & BFJ_COL_SYM) {
+                    in_j = in_i;
+                    band_j = band_i;
+                }
             }
             add_counts(a, c.row, s, i, in_i, lane);
             add_counts(a, (uint32_t)s.row_off + j, s, j, in_j, lane);
+            if (a.emit_pairs) append_pairs(a, (in_i & 1u) && i != j, i, j, lane); // records hold i <= j
             // Band pairs to the refine list: one atomic per wavefront.
             const bool band = (band_i | band_j) != 0u;
             const unsigned long long bb = __ballot(band);
@@ -132,23 +149,48 @@ void bfj_launch_classify(const PassArgs &a, hipStream_t st)
     hipLaunchKernelGGL(k_classify, dim3(2048), dim3(BFJ_BLOCK), 0, st, a);
 }
 
-#define RCH 16                 // doubles per row chunk (one 128-byte line)
-#define RPAD (BFJ_WAVE + 1)    // LDS row length: 65 doubles keeps both phases conflict-free
-#define RWAVES 2              // wavefronts per block: 2 x 17 KiB of staged rows
+#define RK 8 // doubles per row per register chunk
 
-__global__ __launch_bounds__(RWAVES * BFJ_WAVE) void k_refine(PassArgs a)
+// Chunk of RK doubles of one row into registers: 16-byte loads when the dataset's
+// rows are 16-byte aligned (al16), 8-byte loads otherwise.
+typedef const __attribute__((address_space(1))) double *bfj_gptr; // global memory: global_load, not flat_load
+typedef const __attribute__((address_space(1))) bfj_d2 *bfj_gptr2;
+
+__device__ inline void load_chunk(bfj_gptr p, bool al16, double (&u)[RK])
 {
-    __shared__ double sq[RWAVES][RCH][RPAD];
-    __shared__ double sc[RWAVES][RCH][RPAD];
-    __shared__ const double *sptr[RWAVES][2 * BFJ_WAVE];
-    __shared__ int sd[RWAVES][BFJ_WAVE];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (al16) {
+#pragma unroll
+        for (int h = 0; h < RK / 2; ++h) {
+            const bfj_d2 w = *(bfj_gptr2)(p + 2 * h);
+            u[2 * h] = w.x;
+            u[2 * h + 1] = w.y;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < RK; ++k) u[k] = p[k];
+    }
+}
+
+// The reference's direct form over one chunk, k ascending.
+__device__ inline double fma_chunk(const double (&u)[RK], const double (&v)[RK], double acc)
+{
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+        const double t = u[k] - v[k];
+        acc = fma(t, t, acc);
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
+{
+    const int lane = threadIdx.x & 63;
     const uint32_t cnt = *a.band_count;
     const uint32_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
-    const uint32_t groups = (m + BFJ_WAVE - 1) / BFJ_WAVE;
-    const uint32_t wave_id = blockIdx.x * RWAVES + w, n_waves = gridDim.x * RWAVES;
-    for (uint32_t g = wave_id; g < groups; g += n_waves) {
-        const uint32_t t = g * BFJ_WAVE + lane;
+    const uint32_t stride = gridDim.x * BFJ_BLOCK;
+    // base is uniform over the wavefront, so every lane takes part in the votes below.
+    for (uint32_t base = blockIdx.x * BFJ_BLOCK + (threadIdx.x & ~63u); base < m; base += stride) {
+        const uint32_t t = base + lane;
         const bool live = t < m;
         BandEntry be;
         be.i = be.j = 0;
@@ -158,38 +200,35 @@ __global__ __launch_bounds__(RWAVES * BFJ_WAVE) void k_refine(PassArgs a)
         const int si = (int)(be.meta >> 16);
         const SetDesc &s = a.sets[si];
         const int d = live ? s.d : 0;
-        sptr[w][2 * lane] = s.x + (int64_t)be.i * s.ld;
-        sptr[w][2 * lane + 1] = s.x + (int64_t)be.j * s.ld;
-        sd[w][lane] = d;
-        int dmax = d;
-        for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o));
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const bfj_gptr p = (bfj_gptr)(s.x + (int64_t)be.i * s.ld);
+        const bfj_gptr q = (bfj_gptr)(s.x + (int64_t)be.j * s.ld);
+        // Each lane runs the sequential FMA chain of its own pair.  Whole chunks go
+        // through two register buffers, the next chunk's loads in flight while the
+        // current one is consumed; the last d % RK values are loaded one by one.
         double acc = 0.0;
-        for (int k0 = 0; k0 < dmax; k0 += RCH) {
-            // Load: segment sg = (pair p, row r) covers k0..k0+RCH-1 of that row;
-            // 4 segments per instruction, 16 lanes each.
-#pragma unroll 4
-            for (int it = 0; it < 2 * BFJ_WAVE / 4; ++it) {
-                const int sg = it * 4 + (lane >> 4);
-                const int p = sg >> 1, r = sg & 1;
-                const int k = k0 + (lane & 15);
-                const double v = (k < sd[w][p]) ? sptr[w][sg][k] : 0.0;
-                if (r == 0) sq[w][lane & 15][p] = v;
-                else sc[w][lane & 15][p] = v;
+        const int nfull = d / RK;
+        double u0[RK], v0[RK], u1[RK], v1[RK];
+        const bool al16 = s.al16 != 0;
+        if (nfull > 0) {
+            load_chunk(p, al16, u0);
+            load_chunk(q, al16, v0);
+        }
+        for (int c = 0; c < nfull; c += 2) {
+            if (c + 1 < nfull) {
+                load_chunk(p + (c + 1) * RK, al16, u1);
+                load_chunk(q + (c + 1) * RK, al16, v1);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int kend = min(RCH, d - k0);
-            for (int kk = 0; kk < kend; ++kk) {
-                const double df = sq[w][kk][lane] - sc[w][kk][lane];
-                acc = fma(df, df, acc);
+            acc = fma_chunk(u0, v0, acc);
+            if (c + 1 >= nfull) break;
+            if (c + 2 < nfull) {
+                load_chunk(p + (c + 2) * RK, al16, u0);
+                load_chunk(q + (c + 2) * RK, al16, v0);
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            acc = fma_chunk(u1, v1, acc);
+        }
+        for (int k = nfull * RK; k < d; ++k) {
+            const double tk = p[k] - q[k];
+            acc = fma(tk, tk, acc);
         }
         double err = -1.0;
         // Hits per (row, eps) for both rows of the record; lanes of the same row are
@@ -204,6 +243,7 @@ __global__ __launch_bounds__(RWAVES * BFJ_WAVE) void k_refine(PassArgs a)
         }
         add_counts(a, (uint32_t)s.row_off + be.i, s, be.i, hits_i, lane);
         add_counts(a, (uint32_t)s.row_off + be.j, s, be.j, hits_j, lane);
+        if (a.emit_pairs) append_pairs(a, (hits_i & 1u) && be.i != be.j, be.i, be.j, lane);
         if (live) {
             const double sigma = a.scal[si].sigma;
             const double approx = sigma * sigma * (a.he[s.row_off + be.i].x - 2.0 * (double)be.z);
@@ -221,9 +261,25 @@ __global__ __launch_bounds__(RWAVES * BFJ_WAVE) void k_refine(PassArgs a)
     }
 }
 
+// Pair-list mode when eps covers every pair: all (i, j), i < j, row i's pairs at
+// offset i (n - 1) - i (i - 1) / 2, in order.
+__global__ void k_all_pairs(uint2 *pairs, int64_t n)
+{
+    for (int64_t i = blockIdx.x; i < n - 1; i += gridDim.x) {
+        const int64_t off = i * (n - 1) - i * (i - 1) / 2;
+        for (int64_t j = i + 1 + threadIdx.x; j < n; j += blockDim.x)
+            pairs[off + (j - i - 1)] = make_uint2((uint32_t)i, (uint32_t)j);
+    }
+}
+
+void bfj_launch_all_pairs(uint2 *pairs, int64_t n, hipStream_t st)
+{
+    if (n > 1) hipLaunchKernelGGL(k_all_pairs, dim3(4096), dim3(BFJ_BLOCK), 0, st, pairs, n);
+}
+
 void bfj_launch_refine(const PassArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_refine, dim3(4096), dim3(RWAVES * BFJ_WAVE), 0, st, a);
+    hipLaunchKernelGGL(k_refine, dim3(4096), dim3(BFJ_BLOCK), 0, st, a);
 }
 
 // Per-row counts (self excluded) into the caller's int64 array and the
@@ -244,6 +300,8 @@ __global__ void k_finalize(PassArgs a)
             // k_thresholds); the join did not count them.
             const bool all = (a.scal[si].all_mask >> e) & 1ull;
             c = all ? (unsigned long long)s.n : a.cnt[s.cnt_off + (int64_t)e * s.npad + i];
+            // A symmetric sweep streams no self pairs: d2(i, i) == 0 is within every eps >= 0.
+            if (!all && s.sym && s.T[e] >= 0.0) c += 1;
             const long long self = (s.T[e] >= 0.0) ? 1 : 0; // d2(i, i) == 0 exactly
             if (s.out) s.out[(int64_t)e * rows + (i - s.row_begin)] = (long long)c - self;
         }

@@ -226,3 +226,52 @@ def test_cli_drop_in(tmp_path, mode, dtype):
     f = [v.strip() for v in rows[1].split(",")]
     assert int(f[2]) == mode and int(f[4]) == gold["totals"][k] and f[11] == dtype
     assert [int(v) for v in counts.read_text().split()] == gold["counts"][k]
+
+
+def _oracle_pairs(x, eps):
+    """Every unordered pair i < j with d2 <= T(eps), from the oracle's direct form."""
+    n = x.shape[0]
+    iu, ju = np.triu_indices(n, 1)
+    d2 = oracle.pair_d2(x, iu, ju)
+    keep = d2 <= oracle.threshold(eps)
+    return set(zip(iu[keep].tolist(), ju[keep].tolist()))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_pair_list_matches_oracle(ctx, dtype):
+    x = wl.gmm(1800, 24, 8, 0.05, 31)
+    for q in (0.002, 0.03):
+        eps = float(wl.eps_quantile(x, [q], 300, 5)[0])
+        pairs, total, st = ctx.self_join_pairs(x, eps, dtype=dtype)
+        got = set(map(tuple, pairs.astype(np.int64).tolist()))
+        assert len(got) == pairs.shape[0], "duplicate pairs"
+        assert all(i < j for i, j in got)
+        assert got == _oracle_pairs(x, eps)
+        assert total == x.shape[0] + 2 * pairs.shape[0]
+        res = ctx.self_join(x, [eps], dtype=dtype)
+        assert int(res.totals[0]) == total
+
+
+@pytest.mark.parametrize("dtype", ["fp64", "fp16"])
+def test_pair_list_edges(ctx, dtype):
+    x = wl.uniform(700, 9, 77)
+    x[350:360] = x[0]  # exact duplicates: distance 0 pairs
+    # eps < 0: no pair; eps = 0: only the duplicates; eps beyond the diameter: all pairs.
+    pairs, total, _ = ctx.self_join_pairs(x, -1.0, dtype=dtype)
+    assert pairs.shape[0] == 0 and total == 0
+    pairs, total, _ = ctx.self_join_pairs(x, 0.0, dtype=dtype)
+    assert set(map(tuple, pairs.astype(np.int64).tolist())) == _oracle_pairs(x, 0.0)
+    pairs, total, _ = ctx.self_join_pairs(x, 1e6, dtype=dtype, capacity=10)  # grown after CAPACITY
+    n = x.shape[0]
+    assert pairs.shape[0] == n * (n - 1) // 2 and total == n * n
+    iu, ju = np.triu_indices(n, 1)
+    np.testing.assert_array_equal(pairs[:, 0], iu)
+    np.testing.assert_array_equal(pairs[:, 1], ju)
+    # The raw call with a small buffer reports CAPACITY and the number of pairs it needs.
+    import ctypes
+    xc = np.ascontiguousarray(x)
+    buf = np.zeros((5, 2), dtype=np.uint32)
+    found, tot = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    rc = ctx.lib.bfj_self_join_pairs(ctx._h, xc.ctypes.data, n, xc.shape[1], xc.shape[1], 1e6, bfj.DTYPES[dtype],
+                                     buf.ctypes.data, 5, ctypes.byref(found), ctypes.byref(tot), None)
+    assert rc == bfj.BFJ_ERR_CAPACITY and found.value == n * (n - 1) // 2
