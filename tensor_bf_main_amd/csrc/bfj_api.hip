@@ -130,14 +130,14 @@ static int lds_for(int dtype, int kc, int rows, int n_eps)
     return 0;
 }
 
-// Default tile: the most resident blocks per CU (counted up to 3: latency hiding
+// Default tile: the most resident blocks per CU (counted up to 4: latency hiding
 // across blocks matters more than strip width), then the widest strip.
 static int heuristic_rows(int dtype, int kc, int n_eps)
 {
     int best = 0, best_occ = -1;
     for (int r : kRowOptions)
         if (bfj_join_has(dtype, kc, r)) {
-            const int occ = std::min(3, bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps)));
+            const int occ = std::min(4, bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps)));
             if (occ >= best_occ) {
                 best = r;
                 best_occ = occ;
@@ -320,23 +320,34 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         g.rows = rows;
         g.lds = lds_for(dtype, kc, rows, n_eps_max);
         g.item0 = (int32_t)p.items.size();
-        std::vector<std::pair<int64_t, WorkItem>> its;
-        for (size_t si = 0; si < p.sets.size(); ++si) {
+        // Items: one block strip (4 wavefronts x R rows) against one panel of about
+        // 1 MiB of candidate fragments.  Items run panel by panel, so the blocks in
+        // flight on an XCD sweep the same panel and its tiles are fetched into that
+        // XCD's L2 once instead of once per strip.  Datasets largest first.
+        const int64_t tile_bytes = (int64_t)kc * BFJ_WAVE * FB;
+        const int panel = (int)std::max<int64_t>(32, (1 << 20) / tile_bytes);
+        const int strip = (BFJ_BLOCK / BFJ_WAVE) * rows / 16;
+        std::vector<size_t> order;
+        for (size_t si = 0; si < p.sets.size(); ++si)
+            if (p.sets[si].kc == kc && p.sets[si].row_end > p.sets[si].row_begin) order.push_back(si);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return p.sets[x].npad > p.sets[y].npad; });
+        for (size_t si : order) {
             const SetDesc &s = p.sets[si];
-            if (s.kc != kc || s.row_end <= s.row_begin) continue;
-            const int64_t rb_begin = s.row_begin / 16, rb_end = (s.row_end + 15) / 16;
-            // One item = one block strip: 4 wavefronts x R rows, sweeping all candidate
-            // blocks, or (symmetric sweep) those from its first block on.
-            for (int64_t rb = rb_begin; rb < rb_end; rb += (BFJ_BLOCK / BFJ_WAVE) * rows / 16) {
-                WorkItem w;
-                w.set = (int32_t)si;
-                w.rb0 = (int32_t)rb;
-                its.push_back({s.npad / 16 - (s.sym ? rb : 0), w});
+            const int nblk = (int)(s.npad / 16);
+            const int rb_begin = (int)(s.row_begin / 16), rb_end = (int)((s.row_end + 15) / 16);
+            for (int c0 = 0; c0 < nblk; c0 += panel) {
+                const int c1 = std::min(nblk, c0 + panel);
+                for (int rb = rb_begin; rb < rb_end; rb += strip) {
+                    WorkItem w;
+                    w.set = (int32_t)si;
+                    w.rb0 = rb;
+                    w.cb0 = s.sym ? std::max(c0, rb) : c0;
+                    w.cb1 = c1;
+                    if (w.cb0 < w.cb1) p.items.push_back(w);
+                }
             }
         }
-        std::stable_sort(its.begin(), its.end(), [](const std::pair<int64_t, WorkItem> &a,
-                                                    const std::pair<int64_t, WorkItem> &b) { return a.first > b.first; });
-        for (auto &x : its) p.items.push_back(x.second);
         g.n_items = (int32_t)(p.items.size() - g.item0);
         p.max_rows = std::max(p.max_rows, rows);
         if (g.n_items) p.groups.push_back(g);
@@ -350,14 +361,16 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     const int dt = p.dtype;
     const size_t vb = bfj_acc_bytes(dt);
     const size_t nrows = (size_t)std::max<int64_t>(p.rows_total, 1);
-    RC(ensure(ctx->pack, std::max<int64_t>(p.pack_bytes, 64)));
-    RC(ensure(ctx->nq, nrows * vb));
+    // The join reads whole ring slots of up to 4 candidate blocks: the last dataset's
+    // final slot may run past its blocks into this padding (never used).
+    RC(ensure(ctx->pack, (size_t)p.pack_bytes + 65536));
+    RC(ensure(ctx->nq, (nrows + 64) * vb));
     RC(ensure(ctx->gq, nrows * vb));
     RC(ensure(ctx->he, nrows * 16));
     RC(ensure(ctx->row_set, nrows * 2));
     RC(ensure(ctx->qerr, nrows * 8));
     RC(ensure(ctx->nrm, nrows * 8));
-    RC(ensure(ctx->scales, nrows));
+    RC(ensure(ctx->scales, nrows + 64));
     RC(ensure(ctx->cnt, std::max<int64_t>(p.cnt_total, 1) * 4));
     if (ctx->band_capacity == 0) {
         const uint64_t want = std::min<uint64_t>(std::max<uint64_t>((uint64_t)p.rows_total * 64, 1u << 20), 1u << 28);

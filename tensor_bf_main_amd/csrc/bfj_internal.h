@@ -75,10 +75,13 @@ struct BandEntry {
     uint32_t meta;  // bits 0..7: eps to decide for row i; 8..15: for row j; 16..31: set index
 };
 
-// A join work item: one strip of R query rows of one dataset.
+// A join work item: one strip of 4R query rows of one dataset against one panel of
+// candidate blocks [cb0, cb1).
 struct WorkItem {
     int32_t set;
     int32_t rb0; // first 16-row block of the strip
+    int32_t cb0; // candidate blocks [cb0, cb1)
+    int32_t cb1;
 };
 
 // A pair the join could not reject (Z >= 0 for the widest banded eps).  The join

@@ -92,19 +92,23 @@ struct JoinLds {
     typedef typename Tr::val_t val_t;
     static constexpr int FB = (int)sizeof(typename Tr::frag_t);
     static constexpr int TILE = KC * BFJ_WAVE * FB; // candidate fragments of one 16-row block
-    static constexpr int UNITS = TILE / 16;         // 16-byte units of a tile
-    // Direct-to-LDS loads per wavefront per tile: the fragment area is padded to a
+    // A ring slot holds CPT consecutive candidate blocks (about 8 KiB of fragments):
+    // one wait, one barrier and one set of loads per slot instead of per block.
+    static constexpr int CPT_RAW = 8192 / TILE;
+    static constexpr int CPT = CPT_RAW < 1 ? 1 : (CPT_RAW > 4 ? 4 : CPT_RAW);
+    static constexpr int UNITS = CPT * TILE / 16; // 16-byte units of a slot
+    // Direct-to-LDS loads per wavefront per slot: the fragment area is padded to a
     // whole number of block-wide 16-byte loads so every wavefront issues the same
     // count, which keeps the vmcnt bookkeeping a compile-time constant.
     static constexpr int UPW = (UNITS + BFJ_BLOCK - 1) / BFJ_BLOCK;
     static constexpr int FRAG = UPW * BFJ_BLOCK * 16;
-    static constexpr int NQB = 16 * (int)sizeof(val_t); // candidate norms of the tile
-    static constexpr int BUF = FRAG + NQB + 16;          // + FP8 scales
-    // Ring depth: about 16 KiB of tiles per block, at least double buffering.  A
-    // deeper ring buys little; the LDS it would take is worth more as occupancy.
-    static constexpr int NB_RAW = (16 * 1024) / BUF;
+    static constexpr int NQB = CPT * 16 * (int)sizeof(val_t); // candidate norms of the slot
+    static constexpr int BUF = FRAG + NQB + CPT * 16;          // + FP8 scales
+    // Ring depth: about 24 KiB per block, at least double buffering.  Measured: a
+    // deeper ring buys less than the resident blocks its LDS would cost.
+    static constexpr int NB_RAW = (24 * 1024) / BUF;
     static constexpr int NB = NB_RAW < 2 ? 2 : (NB_RAW > 16 ? 16 : NB_RAW);
-    static constexpr int LPT = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0); // DMA instructions per tile
+    static constexpr int LPT = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0); // DMA instructions per slot
     static constexpr int RING = NB * BUF;
     static constexpr int RB = R / 16;
     // Per-wavefront staging of a hot tile's accumulators (lane-private, so the
@@ -174,7 +178,8 @@ __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &c
 }
 
 template <class Tr, int KC, int R>
-__device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, int rb0, unsigned char *lds,
+__device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, int rb0, int cb0, int cb1,
+                                           unsigned char *lds,
                                            CandCursor &cc)
 {
     typedef typename Tr::frag_t frag_t;
@@ -220,26 +225,29 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     const unsigned char *srcb = reinterpret_cast<const unsigned char *>(P);
     const unsigned char *nqb = reinterpret_cast<const unsigned char *>(nq);
     constexpr int NQW = L::NQB / BFJ_JWAVES; // norm bytes per wavefront
-    uint32_t voff[L::UPW];                   // this lane's byte offset in a tile, per load
+    uint32_t voff[L::UPW];                   // this lane's byte offset in a slot, per load
 #pragma unroll
     for (int u = 0; u < L::UPW; ++u) {
         const int unit = u * BFJ_BLOCK + wave * BFJ_WAVE + lane;
         voff[u] = (uint32_t)(unit < L::UNITS ? unit : 0) * 16u;
     }
-    // t and slot are uniform, so the addresses are a scalar base plus a lane offset.
+    // Slot t of the item holds blocks cb0 + t*CPT ... + CPT-1; t and slot are uniform,
+    // so the addresses are a scalar base plus a lane offset.  Blocks past the
+    // dataset's last one read padding that the buffers carry for this (never used).
     auto issue = [&](int t, int slot) {
         unsigned char *b = lds + slot * L::BUF;
-        const unsigned char *gt = srcb + (size_t)t * L::TILE;
+        const int cbase = cb0 + t * L::CPT;
+        const unsigned char *gt = srcb + (size_t)cbase * L::TILE;
 #pragma unroll
         for (int u = 0; u < L::UPW; ++u)
             __builtin_amdgcn_global_load_lds(gt + voff[u], (lds_void *)(b + (u * BFJ_BLOCK + wave * BFJ_WAVE) * 16), 16,
                                              0, 0);
         if (lane < NQW / 4)
-            __builtin_amdgcn_global_load_lds(nqb + (size_t)t * L::NQB + wave * NQW + lane * 4,
+            __builtin_amdgcn_global_load_lds(nqb + (size_t)cbase * 16 * sizeof(val_t) + wave * NQW + lane * 4,
                                              (lds_void *)(b + L::FRAG + wave * NQW), 4, 0, 0);
-        if (Tr::dtype == BFJ_FP8 && lane == 0)
-            __builtin_amdgcn_global_load_lds(sc + (size_t)t * 16 + wave * 4, (lds_void *)(b + L::FRAG + L::NQB + wave * 4),
-                                             4, 0, 0);
+        if (Tr::dtype == BFJ_FP8 && lane < L::CPT)
+            __builtin_amdgcn_global_load_lds(sc + (size_t)cbase * 16 + wave * 4 * L::CPT + lane * 4,
+                                             (lds_void *)(b + L::FRAG + L::NQB + wave * 4 * L::CPT), 4, 0, 0);
     };
     // vmcnt <= (NB-2)*LPT: of this wavefront's loads only the NB-2 newest tiles
     // may still be in flight, so the oldest one (the tile about to be read) landed.
@@ -251,33 +259,37 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     // outstanding vector-memory loads are the ring's.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // A dataset whose rows are all queries sweeps only candidate blocks from the
-    // strip's first block on: every unordered pair lies in exactly one tile (rb, cb)
-    // with cb >= rb, and that tile decides it for both rows.
-    const int nb = (int)nblk; // n < 2^31, so block indices fit in 32 bits
-    const int cb0 = s.sym ? rb0 : 0;
-    // Tile cb lives in ring slot (cb - cb0) mod NB.
+    // The item's panel [cb0, cb1) of candidate blocks.  (A dataset whose rows are all
+    // queries only has panels from the strip's first block on: every unordered pair
+    // lies in exactly one tile (rb, cb) with cb >= rb, which decides it for both rows.)
+    // Item slot t lives in ring buffer t mod NB; loads past the panel re-read its
+    // last slot.
+    const int nt = (cb1 - cb0 + L::CPT - 1) / L::CPT;
 #pragma unroll
-    for (int t = 0; t < L::NB - 1; ++t) issue(min(cb0 + t, nb - 1), t);
-    int rs = 0; // slot of tile cb
-    for (int cb = cb0; cb < nb; ++cb) {
+    for (int t = 0; t < L::NB - 1; ++t) issue(min(t, nt - 1), t);
+    int rs = 0; // ring buffer of slot t
+    for (int t = 0; t < nt; ++t) {
         __builtin_amdgcn_s_waitcnt(WAIT_IMM);
-        // All quarters of tile cb landed (each wavefront waited for its own loads
-        // above); every wavefront is done with tile cb-1, whose fragments fed its
+        // All quarters of slot t landed (each wavefront waited for its own loads
+        // above); every wavefront is done with slot t-1, whose fragments fed its
         // MFMAs before it got here.  No release fence: the ring is written only by
         // the direct-to-LDS loads, and a release would drain the ones in flight.
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        issue(min(cb + L::NB - 1, nb - 1), rs == 0 ? L::NB - 1 : rs - 1); // into the slot of tile cb-1
+        issue(min(t + L::NB - 1, nt - 1), rs == 0 ? L::NB - 1 : rs - 1); // into the buffer of slot t-1
         const int rcur = rs;
         rs = rs + 1 == L::NB ? 0 : rs + 1;
-        if (any_rows) {
+        if (!any_rows) continue;
+#pragma unroll
+        for (int h = 0; h < L::CPT; ++h) {
+            const int cb = cb0 + t * L::CPT + h;
+            if (cb >= cb1) break;
             const unsigned char *b = lds + rcur * L::BUF;
             frag_t B[KC];
 #pragma unroll
-            for (int c = 0; c < KC; ++c) B[c] = reinterpret_cast<const frag_t *>(b)[c * BFJ_WAVE + lane];
-            const val_t nj = reinterpret_cast<const val_t *>(b + L::FRAG)[lane & 15];
-            const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + (lane & 15)] : 127;
+            for (int c = 0; c < KC; ++c) B[c] = reinterpret_cast<const frag_t *>(b)[(h * KC + c) * BFJ_WAVE + lane];
+            const val_t nj = reinterpret_cast<const val_t *>(b + L::FRAG)[h * 16 + (lane & 15)];
+            const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + h * 16 + (lane & 15)] : 127;
 
             // acc = g_i + xhat_i . xhat_j; the first MFMA reads g_i as its C operand, so
             // no VALU work precedes the MFMAs.  Z = acc - n_j is only formed for hot
@@ -344,7 +356,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_join(JoinArgs a)
         if (item >= (uint32_t)a.n_items) break;
         const WorkItem w = a.items[item];
         const SetDesc s = a.sets[w.set];
-        join_strip<Tr, KC, R>(a, s, w.rb0, lds_raw, cc);
+        join_strip<Tr, KC, R>(a, s, w.rb0, w.cb0, w.cb1, lds_raw, cc);
     }
     cand_close(a, cc, lane);
 }
