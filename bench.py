@@ -37,10 +37,27 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def roofline(flops, join_ms, dtype):
+def measured_traffic(workload, dtype, tile_rows):
+    """HBM bytes per join launch from the committed rocprofv3 PMC summaries
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a copy of known size), for the
+    configuration they were measured on; None otherwise."""
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    want = f"workload={workload} dtype={dtype} tile_rows={tile_rows} "
+    if not os.path.isdir(prof):
+        return None
+    for name in sorted(os.listdir(prof)):
+        if not name.endswith(".md"):
+            continue
+        for line in open(os.path.join(prof, name)):
+            if line.startswith("machine-readable: " + want):
+                return float(line.split("traffic_bytes=")[1])
+    return None
+
+
+def roofline(flops, join_ms, dtype, traffic=None):
     achieved = flops / (join_ms * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS[dtype], "unit": "TFLOP/s",
-            "frac": achieved / PEAK_TFLOPS[dtype], "traffic": None,
+            "frac": achieved / PEAK_TFLOPS[dtype], "traffic": traffic,
             "kernel": "k_join (MFMA join + fused epsilon epilogue)"}
 
 
@@ -100,9 +117,19 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
     # Algorithmic flops: true d, FMA = 2, one dot product per unordered pair (the
     # symmetric sweep decides both (i, j) and (j, i) from it).
     flops = 2.0 * d * n * (n + 1) / 2
-    out["roofline"] = roofline(flops, float(np.mean(join_ms)), dtype)
+    out["roofline"] = roofline(flops, float(np.mean(join_ms)), dtype,
+                               None if args.n else measured_traffic(args.workload, dtype, int(last.stats["tile_rows"])))
     got = c_main.numpy()
-    mismatches = None
+    # The host-buffer entry point (rows in host memory, one H2D copy per call, counts
+    # copied back): the PCIe-inclusive rate, reported beside the value, never as it.
+    t0 = time.perf_counter()
+    for _ in range(2):
+        host_res = ctx.self_join(x, e, dtype=dtype)
+    t_host = (time.perf_counter() - t0) / 2
+    out["host_buffers_ms_per_step"] = t_host * 1e3
+    out["host_buffers_pairs_per_s"] = pairs / t_host
+    host_mism = int((host_res.counts != got).sum())
+    mismatches = host_mism
     if dtype != "fp64" and not args.skip_fp64:
         c64 = ctx.empty((e.size, n), np.int64)
         log("fp64 warmup")
@@ -112,7 +139,7 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
         for _ in range(args.fp64_steps):
             r64 = step("fp64", c64)
         t64 = (time.perf_counter() - t0) / args.fp64_steps
-        mismatches = int((got != c64.numpy()).sum()) + int((r64.totals != last.totals).sum())
+        mismatches += int((got != c64.numpy()).sum()) + int((r64.totals != last.totals).sum())
         out["fp64_ms_per_step"] = t64 * 1e3
         out["fp64_pairs_per_s"] = pairs / t64
         out["speedup_vs_fp64"] = t64 / t_main
