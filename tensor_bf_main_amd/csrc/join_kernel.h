@@ -85,6 +85,12 @@ struct TrF64 {
 __device__ inline int bfj_lane() { return __lane_id(); }
 
 #define BFJ_JWAVES (BFJ_BLOCK / BFJ_WAVE) // wavefronts per join block
+#ifndef BFJ_SLOT_BYTES
+#define BFJ_SLOT_BYTES 8192 // candidate fragment bytes per ring slot
+#endif
+#ifndef BFJ_RING_BYTES
+#define BFJ_RING_BYTES (24 * 1024) // target LDS bytes of the candidate ring
+#endif
 
 // LDS layout of one join block (bytes), shared by the kernel and the host.
 template <class Tr, int KC, int R>
@@ -94,7 +100,7 @@ struct JoinLds {
     static constexpr int TILE = KC * BFJ_WAVE * FB; // candidate fragments of one 16-row block
     // A ring slot holds CPT consecutive candidate blocks (about 8 KiB of fragments):
     // one wait, one barrier and one set of loads per slot instead of per block.
-    static constexpr int CPT_RAW = 8192 / TILE;
+    static constexpr int CPT_RAW = BFJ_SLOT_BYTES / TILE;
     static constexpr int CPT = CPT_RAW < 1 ? 1 : (CPT_RAW > 4 ? 4 : CPT_RAW);
     static constexpr int UNITS = CPT * TILE / 16; // 16-byte units of a slot
     // Direct-to-LDS loads per wavefront per slot: the fragment area is padded to a
@@ -106,7 +112,7 @@ struct JoinLds {
     static constexpr int BUF = FRAG + NQB + CPT * 16;          // + FP8 scales
     // Ring depth: about 24 KiB per block, at least double buffering.  Measured: a
     // deeper ring buys less than the resident blocks its LDS would cost.
-    static constexpr int NB_RAW = (24 * 1024) / BUF;
+    static constexpr int NB_RAW = BFJ_RING_BYTES / BUF;
     static constexpr int NB = NB_RAW < 2 ? 2 : (NB_RAW > 16 ? 16 : NB_RAW);
     static constexpr int LPT = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0); // DMA instructions per slot
     static constexpr int RING = NB * BUF;

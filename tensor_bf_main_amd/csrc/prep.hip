@@ -10,17 +10,20 @@
 // per-row error band that makes the low-precision decisions rigorous.
 //
 // Band derivation (kernel space; the join kernel computes
-//   Z_ij = MFMA(C0 = g_i - n_j, xhat_i . xhat_j)  ~=  0.5 * (2 g_i + N_i - S_ij),
+//   Z_ij = MFMA(C = g_i; xhat_i . xhat_j) - n_j  ~=  0.5 * (2 g_i + N_i - S_ij),
 //   S_ij = ||xhat_i - xhat_j||^2):
 //   * quantisation: | ||xhat_i - xhat_j|| - ||y_i - y_j|| | <= q_i + q_j,
 //     q_i >= ||y_i - xhat_i|| measured per row, y the exact centred row.
 //   * oracle rounding: the FP64 direct form differs from the exact squared
 //     distance by at most rho = gamma_{d+2}(2^-53) relatively.
-//   * accumulation: |Z - (g_i - 0.5 N_j + xhat_i . xhat_j)| <= E_i / 2 with
-//     E_i = 2 (1.01 K' + 1) u (|g_i| + N_max + sqrt(N_i N_max)), the standard
-//     bound for K' = 2K + 8 round-to-nearest additions in any order at unit
-//     roundoff u (2^-24 for FP32 accumulation, 2^-53 for FP64), plus the
-//     rounding of C0 and of the FP32 norms.
+//   * accumulation: the MFMA forms acc = g_i + xhat_i . xhat_j (g_i is its C
+//     operand) and Z = fl(acc - n_j), n_j = 0.5 fl32(N_j).  Then
+//     |Z - (g_i - 0.5 N_j + xhat_i . xhat_j)| <= E_i / 2 with
+//     E_i = 2 (1.01 K' + 1) u (|g_i| + sqrt(N_i N_max)) + 4 u N_max (+ FP64 norm
+//     terms): the standard bound for K' = 2K + 8 round-to-nearest operations in any
+//     order at unit roundoff u (2^-24 for FP32 accumulation, 2^-53 for FP64) on
+//     partial sums within |g_i| + sum |p_k| <= |g_i| + sqrt(N_i N_max), plus the
+//     final subtraction and the FP32 rounding of n_j.
 //   So S_hat = (2 g_i + N_i) - 2 Z satisfies |S_hat - S_ij| <= E_i, and for a row
 //   k of the pair (k = i or j) and eps e:
 //     certainly within eps   if S_hat + E_i <= L_ke = (sqrt(T_e/(1+rho)) - r_k)^2
@@ -372,13 +375,15 @@ __global__ void k_thresholds(PassArgs a)
         if (!query || Hstar == -INFINITY) {
             g = fp64 ? -1e300 : -1e30;
         } else {
-            // Accumulation bound; |g| is bounded from the previous estimate.
+            // Accumulation bound; |g| is bounded from the previous estimate.  The MFMA
+            // sums g_i and the K products (partial sums within |g| + sqrt(N_i N_max));
+            // n_j enters once, in the final subtraction and its own FP32 rounding.
             const double Kp = 2.0 * K + 8.0;
             const double coef = 2.0 * (1.01 * Kp + 1.0) * u;
             const double cross = sqrt(Ni * nmax);
             double gabs = 0.5 * fabs(Hstar - Ni);
             for (int it = 0; it < 2; ++it) {
-                E = coef * (gabs + nmax + cross) + 4.0 * (K + 4) * u64 * (Ni + nmax + gabs) + K * 0x1p-120;
+                E = coef * (gabs + cross) + 4.0 * u * nmax + 4.0 * (K + 4) * u64 * (Ni + nmax + gabs) + K * 0x1p-120;
                 gabs = 0.5 * fabs(Hstar - Ni) + E;
             }
             E *= 1.01;
