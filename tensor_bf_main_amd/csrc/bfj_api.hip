@@ -272,7 +272,7 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         memset(&s, 0, sizeof s);
         s.x = h.x;
         s.n = h.n;
-        s.npad = (h.n + 15) / 16 * 16;
+        s.npad = (h.n + 63) / 64 * 64; // whole 4-block ring slots: no partial slot at the end
         s.ld = h.ld;
         s.row_begin = h.row_begin;
         s.row_end = h.row_end;
@@ -325,8 +325,8 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         // flight on an XCD sweep the same panel and its tiles are fetched into that
         // XCD's L2 once instead of once per strip.  Datasets largest first.
         const int64_t tile_bytes = (int64_t)kc * BFJ_WAVE * FB;
-        const int panel = (int)std::max<int64_t>(32, (1 << 20) / tile_bytes);
-        const int strip = (BFJ_BLOCK / BFJ_WAVE) * rows / 16;
+        const int panel = (int)std::max<int64_t>(32, (1 << 20) / tile_bytes / 16 * 16);
+        const int strip = BFJ_JWAVES * rows / 16;
         std::vector<size_t> order;
         for (size_t si = 0; si < p.sets.size(); ++si)
             if (p.sets[si].kc == kc && p.sets[si].row_end > p.sets[si].row_begin) order.push_back(si);
@@ -342,7 +342,10 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
                     WorkItem w;
                     w.set = (int32_t)si;
                     w.rb0 = rb;
-                    w.cb0 = s.sym ? std::max(c0, rb) : c0;
+                    // Panels, strips and nblk are multiples of 4 blocks, so an item is whole
+                    // ring slots (a slot holds 1, 2 or 4 blocks).  Blocks left of the
+                    // diagonal that this adds are dropped by the kernel.
+                    w.cb0 = s.sym ? std::max(c0, rb & ~3) : c0;
                     w.cb1 = c1;
                     if (w.cb0 < w.cb1) p.items.push_back(w);
                 }
@@ -361,16 +364,16 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     const int dt = p.dtype;
     const size_t vb = bfj_acc_bytes(dt);
     const size_t nrows = (size_t)std::max<int64_t>(p.rows_total, 1);
-    // The join reads whole ring slots of up to 4 candidate blocks: the last dataset's
-    // final slot may run past its blocks into this padding (never used).
-    RC(ensure(ctx->pack, (size_t)p.pack_bytes + 65536));
-    RC(ensure(ctx->nq, (nrows + 64) * vb));
+    // The join prefetches up to NB-1 ring slots past the end of an item: after the
+    // last dataset that lands in this padding (never used).
+    RC(ensure(ctx->pack, (size_t)p.pack_bytes + (256 << 10)));
+    RC(ensure(ctx->nq, (nrows + 1024) * vb));
     RC(ensure(ctx->gq, nrows * vb));
     RC(ensure(ctx->he, nrows * 16));
     RC(ensure(ctx->row_set, nrows * 2));
     RC(ensure(ctx->qerr, nrows * 8));
     RC(ensure(ctx->nrm, nrows * 8));
-    RC(ensure(ctx->scales, nrows + 64));
+    RC(ensure(ctx->scales, nrows + 1024));
     RC(ensure(ctx->cnt, std::max<int64_t>(p.cnt_total, 1) * 4));
     if (ctx->band_capacity == 0) {
         const uint64_t want = std::min<uint64_t>(std::max<uint64_t>((uint64_t)p.rows_total * 64, 1u << 20), 1u << 28);

@@ -19,6 +19,10 @@
 #define BFJ_MAXEPS 8
 #define BFJ_WAVE 64
 #define BFJ_BLOCK 256
+#ifndef BFJ_JWAVES
+#define BFJ_JWAVES 4 // wavefronts per join block (they share the candidate ring)
+#endif
+#define BFJ_JBLOCK (BFJ_JWAVES * BFJ_WAVE)
 
 typedef _Float16 bfj_h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bfj_b8 __attribute__((ext_vector_type(8)));
@@ -34,7 +38,7 @@ enum { BFJ_FP64 = 0, BFJ_FP16 = 1, BFJ_BF16 = 2, BFJ_FP8 = 3 };
 struct SetDesc {
     const double *x;   // FP64 input rows (device)
     int64_t n;         // points
-    int64_t npad;      // n rounded up to 16
+    int64_t npad;      // n rounded up to 64 (whole 4-block ring slots)
     int64_t ld;        // row stride of x in doubles
     int64_t row_begin; // query rows [row_begin, row_end)
     int64_t row_end;

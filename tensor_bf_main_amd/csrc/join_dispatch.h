@@ -16,7 +16,7 @@ int BFJ_CAT(bfj_join_launch_, BFJ_SFX)(const JoinLaunch &L, const JoinArgs &a, h
         if (hipFuncSetAttribute((const void *)k_join<BFJ_TR, KC_, R_>,                                        \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, L.lds_bytes) != hipSuccess)      \
             return 2;                                                                                         \
-        hipLaunchKernelGGL((k_join<BFJ_TR, KC_, R_>), dim3(grid), dim3(BFJ_BLOCK), L.lds_bytes, s, a);        \
+        hipLaunchKernelGGL((k_join<BFJ_TR, KC_, R_>), dim3(grid), dim3(BFJ_JBLOCK), L.lds_bytes, s, a);        \
         return hipGetLastError() == hipSuccess ? 0 : 2;                                                       \
     }
     BFJ_LIST(BFJ_X)
@@ -50,7 +50,7 @@ int BFJ_CAT(bfj_join_occ_, BFJ_SFX)(int kc, int rows, int lds_bytes)
         if (hipFuncSetAttribute((const void *)k_join<BFJ_TR, KC_, R_>,                                   \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes) != hipSuccess)   \
             return 0;                                                                                    \
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_join<BFJ_TR, KC_, R_>), BFJ_BLOCK,     \
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_join<BFJ_TR, KC_, R_>), BFJ_JBLOCK,     \
                                                          lds_bytes) != hipSuccess)                      \
             return 0;                                                                                    \
         return nb;                                                                                       \

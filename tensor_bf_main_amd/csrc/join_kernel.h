@@ -84,7 +84,6 @@ struct TrF64 {
 
 __device__ inline int bfj_lane() { return __lane_id(); }
 
-#define BFJ_JWAVES (BFJ_BLOCK / BFJ_WAVE) // wavefronts per join block
 #ifndef BFJ_SLOT_BYTES
 #define BFJ_SLOT_BYTES 8192 // candidate fragment bytes per ring slot
 #endif
@@ -101,20 +100,24 @@ struct JoinLds {
     // A ring slot holds CPT consecutive candidate blocks (about 8 KiB of fragments):
     // one wait, one barrier and one set of loads per slot instead of per block.
     static constexpr int CPT_RAW = BFJ_SLOT_BYTES / TILE;
-    static constexpr int CPT = CPT_RAW < 1 ? 1 : (CPT_RAW > 4 ? 4 : CPT_RAW);
+    // (FP64 slots hold at most 2 blocks, so a slot's norms fit one 64-lane load.)
+    static constexpr int CPT_MAX = (int)sizeof(val_t) == 8 ? 2 : 4;
+    static constexpr int CPT = CPT_RAW < 1 ? 1 : (CPT_RAW > CPT_MAX ? CPT_MAX : CPT_RAW);
     static constexpr int UNITS = CPT * TILE / 16; // 16-byte units of a slot
     // Direct-to-LDS loads per wavefront per slot: the fragment area is padded to a
     // whole number of block-wide 16-byte loads so every wavefront issues the same
     // count, which keeps the vmcnt bookkeeping a compile-time constant.
-    static constexpr int UPW = (UNITS + BFJ_BLOCK - 1) / BFJ_BLOCK;
-    static constexpr int FRAG = UPW * BFJ_BLOCK * 16;
+    static constexpr int UPW = (UNITS + BFJ_JBLOCK - 1) / BFJ_JBLOCK;
+    static constexpr int FRAG = UPW * BFJ_JBLOCK * 16;
     static constexpr int NQB = CPT * 16 * (int)sizeof(val_t); // candidate norms of the slot
     static constexpr int BUF = FRAG + NQB + CPT * 16;          // + FP8 scales
     // Ring depth: about 24 KiB per block, at least double buffering.  Measured: a
     // deeper ring buys less than the resident blocks its LDS would cost.
     static constexpr int NB_RAW = BFJ_RING_BYTES / BUF;
     static constexpr int NB = NB_RAW < 2 ? 2 : (NB_RAW > 16 ? 16 : NB_RAW);
-    static constexpr int LPT = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0); // DMA instructions per slot
+    // DMA instructions per slot: every wavefront issues UPW fragment loads; wavefront 0
+    // also loads the norms and the last wavefront the FP8 scales.
+    static constexpr int LPT_MAX = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0);
     static constexpr int RING = NB * BUF;
     static constexpr int RB = R / 16;
     // Per-wavefront staging of a hot tile's accumulators (lane-private, so the
@@ -223,18 +226,20 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     }
 
     // Candidate tiles stream through an NB-deep LDS ring filled by direct-to-LDS
-    // loads (global_load_lds, no VGPRs), NB-1 tiles ahead of the MFMAs.  Wavefront
-    // w loads units w*64 + 256u + lane of each tile (units past the tile re-read
-    // unit 0 into the padding), a quarter of the norms, and (FP8) a quarter of the
-    // scales: LPT instructions per tile for every wavefront.
+    // loads (global_load_lds, no VGPRs), NB-1 slots ahead of the MFMAs.  Wavefront
+    // w loads units w*64 + JBLOCK*u + lane of each slot (units past the slot re-read
+    // unit 0 into the padding); wavefront 0 also loads the slot's norms and (FP8)
+    // the last wavefront its scales.
     typedef __attribute__((address_space(3))) void lds_void;
     const unsigned char *srcb = reinterpret_cast<const unsigned char *>(P);
     const unsigned char *nqb = reinterpret_cast<const unsigned char *>(nq);
-    constexpr int NQW = L::NQB / BFJ_JWAVES; // norm bytes per wavefront
+    static_assert(L::NQB <= 4 * BFJ_WAVE, "one 64-lane load carries a slot's norms");
+    const bool nq_wave = wave == 0;
+    const bool sc_wave = Tr::dtype == BFJ_FP8 && wave == BFJ_JWAVES - 1;
     uint32_t voff[L::UPW];                   // this lane's byte offset in a slot, per load
 #pragma unroll
     for (int u = 0; u < L::UPW; ++u) {
-        const int unit = u * BFJ_BLOCK + wave * BFJ_WAVE + lane;
+        const int unit = u * BFJ_JBLOCK + wave * BFJ_WAVE + lane;
         voff[u] = (uint32_t)(unit < L::UNITS ? unit : 0) * 16u;
     }
     // Slot t of the item holds blocks cb0 + t*CPT ... + CPT-1; t and slot are uniform,
@@ -246,50 +251,53 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         const unsigned char *gt = srcb + (size_t)cbase * L::TILE;
 #pragma unroll
         for (int u = 0; u < L::UPW; ++u)
-            __builtin_amdgcn_global_load_lds(gt + voff[u], (lds_void *)(b + (u * BFJ_BLOCK + wave * BFJ_WAVE) * 16), 16,
+            __builtin_amdgcn_global_load_lds(gt + voff[u], (lds_void *)(b + (u * BFJ_JBLOCK + wave * BFJ_WAVE) * 16), 16,
                                              0, 0);
-        if (lane < NQW / 4)
-            __builtin_amdgcn_global_load_lds(nqb + (size_t)cbase * 16 * sizeof(val_t) + wave * NQW + lane * 4,
-                                             (lds_void *)(b + L::FRAG + wave * NQW), 4, 0, 0);
-        if (Tr::dtype == BFJ_FP8 && lane < L::CPT)
-            __builtin_amdgcn_global_load_lds(sc + (size_t)cbase * 16 + wave * 4 * L::CPT + lane * 4,
-                                             (lds_void *)(b + L::FRAG + L::NQB + wave * 4 * L::CPT), 4, 0, 0);
+        if (nq_wave && lane < L::NQB / 4)
+            __builtin_amdgcn_global_load_lds(nqb + (size_t)cbase * 16 * sizeof(val_t) + lane * 4,
+                                             (lds_void *)(b + L::FRAG), 4, 0, 0);
+        if (sc_wave && lane < 4 * L::CPT)
+            __builtin_amdgcn_global_load_lds(sc + (size_t)cbase * 16 + lane * 4, (lds_void *)(b + L::FRAG + L::NQB), 4,
+                                             0, 0);
     };
     // vmcnt <= (NB-2)*LPT: of this wavefront's loads only the NB-2 newest tiles
     // may still be in flight, so the oldest one (the tile about to be read) landed.
-    constexpr int WAIT_N = (L::NB - 2) * L::LPT;
-    constexpr int WAIT_IMM = (WAIT_N & 15) | ((WAIT_N >> 4) << 14) | (7 << 4) | (15 << 8);
-    static_assert(WAIT_N < 64, "vmcnt field");
+    // The count depends on how many loads this wavefront issues per slot (uniform).
+    static_assert((L::NB - 2) * L::LPT_MAX < 64, "vmcnt field");
+    const int lpt_extra = (nq_wave ? 1 : 0) + (sc_wave ? 1 : 0);
+    auto wait_ring = [&]() {
+        constexpr int N0 = (L::NB - 2) * L::UPW, N1 = (L::NB - 2) * (L::UPW + 1), N2 = (L::NB - 2) * (L::UPW + 2);
+        if (N0 == N2 || lpt_extra == 0) __builtin_amdgcn_s_waitcnt((N0 & 15) | ((N0 >> 4) << 14) | (7 << 4) | (15 << 8));
+        else if (lpt_extra == 1) __builtin_amdgcn_s_waitcnt((N1 & 15) | ((N1 >> 4) << 14) | (7 << 4) | (15 << 8));
+        else __builtin_amdgcn_s_waitcnt((N2 & 15) | ((N2 >> 4) << 14) | (7 << 4) | (15 << 8));
+    };
 
     // Drain the strip's own loads here, so that inside the loop the only
     // outstanding vector-memory loads are the ring's.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // The item's panel [cb0, cb1) of candidate blocks.  (A dataset whose rows are all
-    // queries only has panels from the strip's first block on: every unordered pair
-    // lies in exactly one tile (rb, cb) with cb >= rb, which decides it for both rows.)
-    // Item slot t lives in ring buffer t mod NB; loads past the panel re-read its
-    // last slot.
-    const int nt = (cb1 - cb0 + L::CPT - 1) / L::CPT;
+    // The item's panel [cb0, cb1) of candidate blocks, a whole number of ring slots.
+    // (A dataset whose rows are all queries only has panels from about the strip's
+    // first block on: every unordered pair lies in exactly one tile (rb, cb) with
+    // cb >= rb, which decides it for both rows.)  Item slot t lives in ring buffer
+    // t mod NB.  The last NB-1 prefetches run past the item into memory that holds
+    // the next blocks or the buffers' tail padding; they are never used.
+    const int nt = (cb1 - cb0) / L::CPT;
 #pragma unroll
-    for (int t = 0; t < L::NB - 1; ++t) issue(min(t, nt - 1), t);
-    int rs = 0; // ring buffer of slot t
-    for (int t = 0; t < nt; ++t) {
-        __builtin_amdgcn_s_waitcnt(WAIT_IMM);
+    for (int t = 0; t < L::NB - 1; ++t) issue(t, t);
+    auto step = [&](int t, int rcur) {
+        wait_ring();
         // All quarters of slot t landed (each wavefront waited for its own loads
         // above); every wavefront is done with slot t-1, whose fragments fed its
         // MFMAs before it got here.  No release fence: the ring is written only by
         // the direct-to-LDS loads, and a release would drain the ones in flight.
         __builtin_amdgcn_s_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        issue(min(t + L::NB - 1, nt - 1), rs == 0 ? L::NB - 1 : rs - 1); // into the buffer of slot t-1
-        const int rcur = rs;
-        rs = rs + 1 == L::NB ? 0 : rs + 1;
-        if (!any_rows) continue;
+        issue(t + L::NB - 1, rcur == 0 ? L::NB - 1 : rcur - 1); // into the buffer of slot t-1
+        if (!any_rows) return;
 #pragma unroll
         for (int h = 0; h < L::CPT; ++h) {
             const int cb = cb0 + t * L::CPT + h;
-            if (cb >= cb1) break;
             const unsigned char *b = lds + rcur * L::BUF;
             frag_t B[KC];
 #pragma unroll
@@ -336,8 +344,14 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
                 }
             }
         }
+    };
+    // Unrolled by NB so that every ring address is a constant.
+    for (int t0 = 0; t0 < nt; t0 += L::NB) {
+#pragma unroll
+        for (int j = 0; j < L::NB; ++j)
+            if (t0 + j < nt) step(t0 + j, j);
     }
-    // The clamped tail loads must land before the next strip reuses the ring.
+    // The prefetched tail loads must land before the next strip reuses the ring.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 }
@@ -346,7 +360,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 // it is empty.  All waves of a block stay together (the candidate tiles are
 // shared), so the queue is read by one thread and broadcast through LDS.
 template <class Tr, int KC, int R>
-__global__ __launch_bounds__(BFJ_BLOCK) void k_join(JoinArgs a)
+__global__ __launch_bounds__(BFJ_JBLOCK) void k_join(JoinArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     __shared__ uint32_t item_s;

@@ -80,66 +80,93 @@ __device__ inline void decide(const SetScalars &sc, uint32_t band_mask, int n_ep
     }
 }
 
+// The decisions for one candidate record (lanes with live == false decide nothing).
+struct Decided {
+    Candidate c;
+    int si;
+    uint32_t i, j, in_i, band_i, in_j, band_j;
+};
+
+__device__ inline Decided decide_record(const PassArgs &a, const Candidate *rec, bool live)
+{
+    Decided o;
+    o.c.row = 0;
+    o.c.col = 0;
+    o.c.z = -1.0;
+    if (live) o.c = *rec;
+    o.si = live ? (int)a.row_set[o.c.row] : 0;
+    const SetDesc &s = a.sets[o.si];
+    o.i = o.c.row - (uint32_t)s.row_off;
+    o.j = o.c.col & ~BFJ_COL_SYM;
+    o.in_i = o.band_i = o.in_j = o.band_j = 0;
+    if (live) {
+        // S_hat = 2 g_i + N_i - 2 Z, |S_hat - S| <= E_i; mg covers the FP64 rounding
+        // of these few operations.
+        const double2 h = a.he[o.c.row];
+        const double shat = h.x - 2.0 * o.c.z;
+        const double mg = (fabs(h.x) + 2.0 * fabs(o.c.z) + h.y) * 0x1p-50;
+        const double lo = shat - h.y - mg, hi = shat + h.y + mg;
+        // |dist(xhat_i, xhat_j) - dist(y_i, y_j)| <= q_i + q_j: the same bounds hold
+        // for both rows of the pair.
+        const double r = (a.qerr[o.c.row] + a.qerr[s.row_off + o.j]) * (1.0 + 0x1p-50);
+        const SetScalars &sc = a.scal[o.si];
+        decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, lo, hi, o.in_i, o.band_i);
+        if (o.c.col & BFJ_COL_SYM) {
+            o.in_j = o.in_i;
+            o.band_j = o.band_i;
+        }
+    }
+    return o;
+}
+
 __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t n_chunks = min(*a.chunk_count, a.chunk_capacity);
     const uint32_t wave_id = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
     const uint32_t n_waves = gridDim.x * (BFJ_BLOCK / BFJ_WAVE);
     for (uint32_t ch = wave_id; ch < n_chunks; ch += n_waves) {
         const uint32_t fill = min(a.chunk_fill[ch], (uint32_t)BFJ_CHUNK);
-        for (uint32_t t0 = 0; t0 < fill; t0 += BFJ_WAVE) {
-            const uint32_t t = t0 + lane;
-            const bool live = t < fill;
-            Candidate c;
-            c.row = 0;
-            c.col = 0;
-            c.z = -1.0;
-            if (live) c = a.cand[(size_t)ch * BFJ_CHUNK + t];
-            const int si = live ? (int)a.row_set[c.row] : 0;
-            const SetDesc &s = a.sets[si];
-            const uint32_t i = c.row - (uint32_t)s.row_off;
-            const uint32_t j = c.col & ~BFJ_COL_SYM;
-            uint32_t in_i = 0, band_i = 0, in_j = 0, band_j = 0;
-            if (live) {
-                // S_hat = 2 g_i + N_i - 2 Z, |S_hat - S| <= E_i; mg covers the FP64
-                // rounding of these few operations.
-                const double2 h = a.he[c.row];
-                const double shat = h.x - 2.0 * c.z;
-                const double mg = (fabs(h.x) + 2.0 * fabs(c.z) + h.y) * 0x1p-50;
-                const double lo = shat - h.y - mg, hi = shat + h.y + mg;
-                // |dist(xhat_i, xhat_j) - dist(y_i, y_j)| <= q_i + q_j: the same bounds
-                // hold for both rows of the pair.
-                const double r = (a.qerr[c.row] + a.qerr[s.row_off + j]) * (1.0 + 0x1p-50);
-                const SetScalars &sc = a.scal[si];
-                decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, lo, hi, in_i, band_i);
-                if (c.col & BFJ_COL_SYM) {
-                    in_j = in_i;
-                    band_j = band_i;
-                }
-            }
-            add_counts(a, c.row, s, i, in_i, lane);
-            add_counts(a, (uint32_t)s.row_off + j, s, j, in_j, lane);
-            if (a.emit_pairs) append_pairs(a, (in_i & 1u) && i != j, i, j, lane); // records hold i <= j
-            // Band pairs to the refine list: one atomic per wavefront.
-            const bool band = (band_i | band_j) != 0u;
+        const Candidate *recs = a.cand + (size_t)ch * BFJ_CHUNK;
+        // Pass 1: count the certain-in pairs, note which records go to the refine list.
+        uint32_t band_groups = 0, n_band = 0;
+        for (uint32_t t0 = 0, g = 0; t0 < fill; t0 += BFJ_WAVE, ++g) {
+            const bool live = t0 + lane < fill;
+            const Decided d = decide_record(a, recs + t0 + lane, live);
+            const SetDesc &s = a.sets[d.si];
+            add_counts(a, d.c.row, s, d.i, d.in_i, lane);
+            add_counts(a, (uint32_t)s.row_off + d.j, s, d.j, d.in_j, lane);
+            if (a.emit_pairs) append_pairs(a, (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane); // records hold i <= j
+            const uint32_t nb = (uint32_t)__popcll(__ballot((d.band_i | d.band_j) != 0u));
+            if (nb) band_groups |= 1u << g;
+            n_band += nb;
+        }
+        if (!n_band) continue;
+        // One atomic per chunk reserves its refine-list records (a counter that every
+        // wavefront hits: per chunk rather than per 64 records keeps it uncontended).
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(a.band_count, n_band);
+        base = __shfl(base, 0);
+        // Pass 2: decide the band groups again and write their records.
+        for (uint32_t t0 = 0, g = 0; t0 < fill; t0 += BFJ_WAVE, ++g) {
+            if (!((band_groups >> g) & 1u)) continue;
+            const bool live = t0 + lane < fill;
+            const Decided d = decide_record(a, recs + t0 + lane, live);
+            const bool band = (d.band_i | d.band_j) != 0u;
             const unsigned long long bb = __ballot(band);
-            if (bb) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(a.band_count, (uint32_t)__popcll(bb));
-                base = __shfl(base, 0);
-                if (band) {
-                    const uint32_t idx = base + (uint32_t)__popcll(bb & ((1ull << lane) - 1ull));
-                    if (idx < a.band_capacity) {
-                        BandEntry be;
-                        be.i = i;
-                        be.j = j;
-                        be.z = (float)c.z;
-                        be.meta = band_i | (band_j << 8) | ((uint32_t)si << 16);
-                        a.band[idx] = be;
-                    }
+            if (band) {
+                const uint32_t idx = base + (uint32_t)__popcll(bb & below);
+                if (idx < a.band_capacity) {
+                    BandEntry be;
+                    be.i = d.i;
+                    be.j = d.j;
+                    be.z = (float)d.c.z;
+                    be.meta = d.band_i | (d.band_j << 8) | ((uint32_t)d.si << 16);
+                    a.band[idx] = be;
                 }
             }
+            base += (uint32_t)__popcll(bb);
         }
     }
 }
@@ -188,6 +215,11 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
     const uint32_t cnt = *a.band_count;
     const uint32_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
     const uint32_t stride = gridDim.x * BFJ_BLOCK;
+    // Largest approximation error seen by this lane, per dataset: kept in a register
+    // while the dataset does not change, flushed once (one atomic on a counter every
+    // wavefront shares would serialise the whole kernel).
+    int err_set = -1;
+    double err_run = -1.0;
     // base is uniform over the wavefront, so every lane takes part in the votes below.
     for (uint32_t base = blockIdx.x * BFJ_BLOCK + (threadIdx.x & ~63u); base < m; base += stride) {
         const uint32_t t = base + lane;
@@ -248,16 +280,24 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
             const double sigma = a.scal[si].sigma;
             const double approx = sigma * sigma * (a.he[s.row_off + be.i].x - 2.0 * (double)be.z);
             err = fabs(approx - acc);
+            if (si != err_set) {
+                if (err_set >= 0 && err_run >= 0.0)
+                    atomicMax(&a.scal[err_set].err_bits, (unsigned long long)__double_as_longlong(err_run));
+                err_set = si;
+                err_run = -1.0;
+            }
+            err_run = fmax(err_run, err);
         }
-        // Largest approximation error of the group: one atomic per dataset present.
-        const int si0 = __shfl(si, 0);
-        if (__ballot(live && si != si0) == 0ull) {
-            double mx = err;
-            for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
-            if (lane == 0 && mx >= 0.0) atomicMax(&a.scal[si0].err_bits, (unsigned long long)__double_as_longlong(mx));
-        } else if (err >= 0.0) {
-            atomicMax(&a.scal[si].err_bits, (unsigned long long)__double_as_longlong(err));
-        }
+    }
+    // Flush: one atomic per wavefront for the common case of one dataset.
+    const int set0 = __shfl(err_set, 0);
+    if (__ballot(err_set != set0) == 0ull) {
+        double mx = err_run;
+        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+        if (lane == 0 && set0 >= 0 && mx >= 0.0)
+            atomicMax(&a.scal[set0].err_bits, (unsigned long long)__double_as_longlong(mx));
+    } else if (err_set >= 0 && err_run >= 0.0) {
+        atomicMax(&a.scal[err_set].err_bits, (unsigned long long)__double_as_longlong(err_run));
     }
 }
 
