@@ -283,7 +283,16 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         s.sym = (h.row_begin == 0 && h.row_end == h.n) ? 1 : 0;
         s.al16 = ((uintptr_t)h.x % 16 == 0 && h.ld % 2 == 0) ? 1 : 0;
         s.out = h.out;
-        for (int e = 0; e < h.n_eps; ++e) s.T[e] = eps_threshold(h.eps[e]);
+        // Thresholds in ascending order: a pair within eps k is within every later one,
+        // so the kernels count each pair once, under the first eps that holds it.
+        std::vector<std::pair<double, int>> te;
+        for (int e = 0; e < h.n_eps; ++e) te.push_back({eps_threshold(h.eps[e]), e});
+        std::stable_sort(te.begin(), te.end(),
+                         [](const std::pair<double, int> &x, const std::pair<double, int> &y) { return x.first < y.first; });
+        for (int e = 0; e < h.n_eps; ++e) {
+            s.T[e] = te[e].first;
+            s.eperm[e] = (int8_t)te[e].second;
+        }
         s.row_off = p.rows_total;
         s.pack_off = p.pack_bytes;
         s.cnt_off = p.cnt_total;

@@ -10,7 +10,8 @@
 //   gq[row]    g_i = 0.5 * (H* + E_i - N_i) rounded up (H*: largest banded H of the
 //              dataset); -BIG for rows that are not queries
 //   he[row]    (2 g_i + N_i, E_i): S_hat = he.x - 2 Z estimates S with |S_hat - S| <= E_i
-//   cnt[e][row] per-row hit counts, self included (uint32)
+//   cnt[k][row] per-row pair counts by bucket: pairs whose smallest eps (in ascending
+//              order) that holds them is k; eps e counts buckets 0..e (uint32)
 //   band list  (i, j, z, mask) records of pairs whose band straddles a threshold
 #pragma once
 #include <hip/hip_runtime.h>
@@ -52,7 +53,8 @@ struct SetDesc {
     int32_t set_id;
     int32_t sym;       // every row is a query: each unordered pair is evaluated once
     int32_t al16;      // x and ld keep every row 16-byte aligned
-    double T[BFJ_MAXEPS]; // input-space thresholds: hit iff d2 <= T
+    double T[BFJ_MAXEPS]; // input-space thresholds, ascending: hit iff d2 <= T
+    int8_t eperm[BFJ_MAXEPS]; // T[k] belongs to the caller's eps[eperm[k]]
 };
 
 // Per-dataset device scalars produced by the prep kernels.
@@ -76,7 +78,7 @@ struct BandEntry {
     uint32_t i;     // dataset-local row of the tile (the one Z was computed for)
     uint32_t j;     // dataset-local candidate row
     float z;        // Z value of the pair from the join kernel (kernel space)
-    uint32_t meta;  // bits 0..7: eps to decide for row i; 8..15: for row j; 16..31: set index
+    uint32_t meta;  // bits 0..7: eps in the band; bit 15: counts for row j too; 16..31: set index
 };
 
 // A join work item: one strip of 4R query rows of one dataset against one panel of

@@ -25,26 +25,20 @@
 // counted, inside the band goes to the refine list, otherwise out.  A record of a
 // symmetric sweep decides for its row and for its candidate row.  Counts are summed
 // per distinct row of the wavefront before one atomic per row and eps.
-// cnt[row][e] += 1 for every lane and every eps bit of `mask`; lanes with the same
-// global row are summed first (a vote loop over the distinct rows of the wavefront).
-// `li` is the dataset-local row of `key` in dataset `s`.  All lanes take part.
-__device__ inline void add_counts(const PassArgs &a, uint32_t key, const SetDesc &s, uint32_t li, uint32_t mask,
-                                  int lane)
+// cnt[bucket][row] += 1 for every lane with a bucket (>= 0); lanes with the same
+// global row and bucket are summed first (a vote loop over the distinct keys of the
+// wavefront).  `li` is the dataset-local row of `key` in dataset `s`.  All lanes take
+// part.
+__device__ inline void add_bucket(const PassArgs &a, uint32_t key, const SetDesc &s, uint32_t li, int bucket, int lane)
 {
-    unsigned long long pending = __ballot(mask != 0u);
-    if (!pending) return;
-    uint32_t any = mask;
-    for (int o = 32; o > 0; o >>= 1) any |= (uint32_t)__shfl_xor((int)any, o);
+    const unsigned long long k = bucket >= 0 ? (((unsigned long long)key << 3) | (unsigned)bucket) : ~0ull;
+    unsigned long long pending = __ballot(bucket >= 0);
     while (pending) {
         const int leader = __ffsll((long long)pending) - 1;
-        const uint32_t k = (uint32_t)__shfl((int)key, leader);
-        const bool same = key == k;
-        for (int e = 0; e < BFJ_MAXEPS; ++e) {
-            if (!((any >> e) & 1u)) continue;
-            const unsigned long long m = __ballot(same && ((mask >> e) & 1u));
-            if (m && lane == leader) atomicAdd(&a.cnt[s.cnt_off + (int64_t)e * s.npad + li], (uint32_t)__popcll(m));
-        }
-        pending &= ~__ballot(same);
+        const unsigned long long kk = __shfl(k, leader);
+        const unsigned long long m = __ballot(k == kk);
+        if (lane == leader) atomicAdd(&a.cnt[s.cnt_off + (int64_t)bucket * s.npad + li], (uint32_t)__popcll(m));
+        pending &= ~m;
     }
 }
 
@@ -135,10 +129,13 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
             const bool live = t0 + lane < fill;
             const Decided d = decide_record(a, recs + t0 + lane, live);
             const SetDesc &s = a.sets[d.si];
-            add_counts(a, d.c.row, s, d.i, d.in_i, lane);
-            add_counts(a, (uint32_t)s.row_off + d.j, s, d.j, d.in_j, lane);
-            if (a.emit_pairs) append_pairs(a, (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane); // records hold i <= j
-            const uint32_t nb = (uint32_t)__popcll(__ballot((d.band_i | d.band_j) != 0u));
+            // A record with an eps in the band is decided whole by the FP64 refine;
+            // otherwise it counts once, under the first eps that certainly holds it.
+            const bool band = (d.band_i | d.band_j) != 0u;
+            add_bucket(a, d.c.row, s, d.i, (!band && d.in_i) ? __builtin_ctz(d.in_i) : -1, lane);
+            add_bucket(a, (uint32_t)s.row_off + d.j, s, d.j, (!band && d.in_j) ? __builtin_ctz(d.in_j) : -1, lane);
+            if (a.emit_pairs) append_pairs(a, !band && (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane); // i <= j
+            const uint32_t nb = (uint32_t)__popcll(__ballot(band));
             if (nb) band_groups |= 1u << g;
             n_band += nb;
         }
@@ -162,7 +159,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
                     be.i = d.i;
                     be.j = d.j;
                     be.z = (float)d.c.z;
-                    be.meta = d.band_i | (d.band_j << 8) | ((uint32_t)d.si << 16);
+                    be.meta = (d.band_i | d.band_j) | ((d.c.col & BFJ_COL_SYM) ? 0x8000u : 0u) | ((uint32_t)d.si << 16);
                     a.band[idx] = be;
                 }
             }
@@ -263,19 +260,18 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
             acc = fma(tk, tk, acc);
         }
         double err = -1.0;
-        // Hits per (row, eps) for both rows of the record; lanes of the same row are
-        // summed first (records of one group share few rows).
-        uint32_t hits_i = 0, hits_j = 0;
-        if (live) {
+        // The pair counts once, under the first eps (ascending) whose threshold holds
+        // its FP64 distance, for its row and (symmetric record) for the other row.
+        int k = -1;
+        if (live)
             for (int e = 0; e < s.n_eps; ++e)
                 if (acc <= s.T[e]) {
-                    hits_i |= be.meta & (1u << e);
-                    hits_j |= (be.meta >> 8) & (1u << e);
+                    k = e;
+                    break;
                 }
-        }
-        add_counts(a, (uint32_t)s.row_off + be.i, s, be.i, hits_i, lane);
-        add_counts(a, (uint32_t)s.row_off + be.j, s, be.j, hits_j, lane);
-        if (a.emit_pairs) append_pairs(a, (hits_i & 1u) && be.i != be.j, be.i, be.j, lane);
+        add_bucket(a, (uint32_t)s.row_off + be.i, s, be.i, k, lane);
+        add_bucket(a, (uint32_t)s.row_off + be.j, s, be.j, (be.meta & 0x8000u) ? k : -1, lane);
+        if (a.emit_pairs) append_pairs(a, k == 0 && be.i != be.j, be.i, be.j, lane);
         if (live) {
             const double sigma = a.scal[si].sigma;
             const double approx = sigma * sigma * (a.he[s.row_off + be.i].x - 2.0 * (double)be.z);
@@ -333,17 +329,19 @@ __global__ void k_finalize(PassArgs a)
     const int64_t i = a.blk_row0[b] + threadIdx.x;
     const bool query = i < s.n && i >= s.row_begin && i < s.row_end;
     const int64_t rows = s.row_end - s.row_begin;
+    unsigned long long run = 0; // buckets 0..e: pairs within the e-th smallest eps
     for (int e = 0; e < s.n_eps; ++e) {
         unsigned long long c = 0;
         if (query) {
+            run += a.cnt[s.cnt_off + (int64_t)e * s.npad + i];
             // "all" eps: every candidate, self included, is within eps (bounded in
             // k_thresholds); the join did not count them.
             const bool all = (a.scal[si].all_mask >> e) & 1ull;
-            c = all ? (unsigned long long)s.n : a.cnt[s.cnt_off + (int64_t)e * s.npad + i];
+            c = all ? (unsigned long long)s.n : run;
             // A symmetric sweep streams no self pairs: d2(i, i) == 0 is within every eps >= 0.
             if (!all && s.sym && s.T[e] >= 0.0) c += 1;
             const long long self = (s.T[e] >= 0.0) ? 1 : 0; // d2(i, i) == 0 exactly
-            if (s.out) s.out[(int64_t)e * rows + (i - s.row_begin)] = (long long)c - self;
+            if (s.out) s.out[(int64_t)s.eperm[e] * rows + (i - s.row_begin)] = (long long)c - self;
         }
         red[e][threadIdx.x] = c;
     }
@@ -351,7 +349,7 @@ __global__ void k_finalize(PassArgs a)
     if (threadIdx.x < (unsigned)s.n_eps) {
         unsigned long long t = 0;
         for (int r = 0; r < BFJ_PREP_ROWS; ++r) t += red[threadIdx.x][r];
-        if (t) atomicAdd(&a.scal[si].total[threadIdx.x], t);
+        if (t) atomicAdd(&a.scal[si].total[s.eperm[threadIdx.x]], t);
     }
 }
 

@@ -124,6 +124,20 @@ def test_multi_eps_equals_single(ctx):
         assert multi.totals[k] == single.totals[0]
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("sym", [True, False])
+def test_unsorted_eps_lists(ctx, dtype, sym):
+    # Eps in no particular order, with repeats, a negative one and one beyond the
+    # diameter: counts come back in the caller's order.
+    x = wl.gmm(3000, 40, 6, 0.05, 91)
+    q = wl.eps_quantile(x, [0.02, 0.001, 0.2], 300, 4)
+    eps = [q[0], -1.0, q[1], 1e3, q[2], q[1], 0.0, q[0]]
+    if sym:
+        _check(ctx, x, eps, dtype)
+    else:
+        _check(ctx, x, eps, dtype, row_begin=700, row_end=2100)
+
+
 @pytest.mark.parametrize("dtype", ["fp16", "fp64"])
 def test_batch_mixed_dims(ctx, dtype):
     sets = wl.batch(n_sets=12, seed=77, n_min=100, n_max=3000)
