@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <memory>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -171,6 +172,11 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+struct Group {
+    int kc, rows, lds;
+    int32_t item0, n_items;
+};
+
 struct bfj_ctx {
     int device = 0;
     int n_cu = 0;
@@ -183,6 +189,13 @@ struct bfj_ctx {
     int emit_pairs = 0;
     uint32_t chunk_capacity = 0; // candidate chunks of BFJ_CHUNK records
     int tile_rows[4][33] = {}; // [dtype][kc] forced/tuned R (0 = heuristic)
+    // Work items of the last plan shape (repeated calls on the same shapes reuse them,
+    // and skip the upload while the device copy is current).
+    std::vector<int64_t> item_key;
+    std::shared_ptr<const std::vector<WorkItem>> item_cache;
+    std::vector<Group> group_cache;
+    const std::vector<WorkItem> *items_staged = nullptr;
+    void *items_staged_buf = nullptr;
 };
 
 // Bumped whenever a device buffer moves: a captured graph holds raw pointers.
@@ -224,15 +237,10 @@ struct HostSet {
     int64_t *out; // device or NULL
 };
 
-struct Group {
-    int kc, rows, lds;
-    int32_t item0, n_items;
-};
-
 struct PassPlan {
     int dtype = 0;
     std::vector<SetDesc> sets;
-    std::vector<WorkItem> items;
+    std::shared_ptr<const std::vector<WorkItem>> items; // shared with the context's cache
     std::vector<Group> groups;
     std::vector<int32_t> blk_set, set_blk0, set_nblk;
     std::vector<int64_t> blk_row0;
@@ -317,8 +325,12 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
     for (auto &s : p.sets)
         if (std::find(kcs.begin(), kcs.end(), s.kc) == kcs.end()) kcs.push_back(s.kc);
     std::sort(kcs.begin(), kcs.end());
+    // Tile shape per K-chunk group, then the items (reused when the shapes repeat).
+    std::vector<Group> groups;
+    std::vector<int64_t> key = {dtype, (int64_t)p.sets.size()};
     for (int kc : kcs) {
         Group g;
+        memset(&g, 0, sizeof g);
         g.kc = kc;
         int n_eps_max = 1;
         for (auto &s : p.sets)
@@ -328,10 +340,23 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         if (!bfj_join_has(dtype, kc, rows)) return fail(BFJ_ERR_UNSUPPORTED, "no tile for dtype %d kc %d", dtype, kc);
         g.rows = rows;
         g.lds = lds_for(dtype, kc, rows, n_eps_max);
-        g.item0 = (int32_t)p.items.size();
-        // Items: one block strip (4 wavefronts x R rows) against one panel of about
-        // 1 MiB of candidate fragments.  Items run panel by panel, so the blocks in
-        // flight on an XCD sweep the same panel and its tiles are fetched into that
+        groups.push_back(g);
+        key.insert(key.end(), {kc, rows, g.lds});
+    }
+    for (auto &s : p.sets) key.insert(key.end(), {s.kc, s.npad, s.sym, s.row_begin, s.row_end});
+    if (ctx->item_cache && key == ctx->item_key) {
+        p.items = ctx->item_cache;
+        p.groups = ctx->group_cache;
+        for (auto &g : p.groups) p.max_rows = std::max(p.max_rows, g.rows);
+        return BFJ_OK;
+    }
+    auto items = std::make_shared<std::vector<WorkItem>>();
+    for (Group &g : groups) {
+        const int kc = g.kc, rows = g.rows;
+        g.item0 = (int32_t)items->size();
+        // Items: one block strip (JWAVES wavefronts x R rows) against one panel of
+        // about 1 MiB of candidate fragments.  Items run panel by panel, so the blocks
+        // in flight on an XCD sweep the same panel and its tiles are fetched into that
         // XCD's L2 once instead of once per strip.  Datasets largest first.
         const int64_t tile_bytes = (int64_t)kc * BFJ_WAVE * FB;
         const int panel = (int)std::max<int64_t>(32, (1 << 20) / tile_bytes / 16 * 16);
@@ -356,14 +381,18 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
                     // diagonal that this adds are dropped by the kernel.
                     w.cb0 = s.sym ? std::max(c0, rb & ~3) : c0;
                     w.cb1 = c1;
-                    if (w.cb0 < w.cb1) p.items.push_back(w);
+                    if (w.cb0 < w.cb1) items->push_back(w);
                 }
             }
         }
-        g.n_items = (int32_t)(p.items.size() - g.item0);
+        g.n_items = (int32_t)(items->size() - g.item0);
         p.max_rows = std::max(p.max_rows, rows);
         if (g.n_items) p.groups.push_back(g);
     }
+    p.items = items;
+    ctx->item_key = key;
+    ctx->item_cache = items;
+    ctx->group_cache = p.groups;
     return BFJ_OK;
 }
 
@@ -402,14 +431,18 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->mu, p.sets.size() * BFJ_MAXDIM * 8));
     RC(ensure(ctx->sets, p.sets.size() * sizeof(SetDesc)));
     RC(ensure(ctx->scal, p.sets.size() * sizeof(SetScalars)));
-    RC(ensure(ctx->items, std::max<size_t>(p.items.size(), 1) * sizeof(WorkItem)));
+    const size_t n_items = p.items ? p.items->size() : 0;
+    RC(ensure(ctx->items, std::max<size_t>(n_items, 1) * sizeof(WorkItem)));
     const size_t nb = p.blk_set.size(), ns = p.sets.size();
     const size_t tab_bytes = nb * 4 + nb * 8 + ns * 4 * 2 + 64;
     RC(ensure(ctx->tabs, tab_bytes));
     HC(hipMemcpyAsync(ctx->sets.p, p.sets.data(), ns * sizeof(SetDesc), hipMemcpyHostToDevice, ctx->stream));
-    if (!p.items.empty())
-        HC(hipMemcpyAsync(ctx->items.p, p.items.data(), p.items.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+    if (n_items && (ctx->items_staged != p.items.get() || ctx->items_staged_buf != ctx->items.p)) {
+        ctx->items_staged = p.items.get();
+        ctx->items_staged_buf = ctx->items.p;
+        HC(hipMemcpyAsync(ctx->items.p, p.items->data(), n_items * sizeof(WorkItem), hipMemcpyHostToDevice,
                           ctx->stream));
+    }
     unsigned char *t = (unsigned char *)ctx->tabs.p;
     HC(hipMemcpyAsync(t, p.blk_row0.data(), nb * 8, hipMemcpyHostToDevice, ctx->stream));
     HC(hipMemcpyAsync(t + nb * 8, p.blk_set.data(), nb * 4, hipMemcpyHostToDevice, ctx->stream));

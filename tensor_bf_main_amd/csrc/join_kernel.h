@@ -87,6 +87,9 @@ __device__ inline int bfj_lane() { return __lane_id(); }
 #ifndef BFJ_SLOT_BYTES
 #define BFJ_SLOT_BYTES 8192 // candidate fragment bytes per ring slot
 #endif
+#ifndef BFJ_INTERLEAVE
+#define BFJ_INTERLEAVE 1 // candidate blocks whose MFMA chains are interleaved
+#endif
 #ifndef BFJ_RING_BYTES
 #define BFJ_RING_BYTES (24 * 1024) // target LDS bytes of the candidate ring
 #endif
@@ -295,52 +298,68 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
         issue(t + L::NB - 1, rcur == 0 ? L::NB - 1 : rcur - 1); // into the buffer of slot t-1
         if (!any_rows) return;
+        const unsigned char *b = lds + rcur * L::BUF;
+        // IL blocks at a time: their MFMA chains interleave (more independent
+        // accumulators in flight) before the max trees read them.
+        constexpr int IL = L::CPT < BFJ_INTERLEAVE ? L::CPT : BFJ_INTERLEAVE;
 #pragma unroll
-        for (int h = 0; h < L::CPT; ++h) {
-            const int cb = cb0 + t * L::CPT + h;
-            const unsigned char *b = lds + rcur * L::BUF;
-            frag_t B[KC];
+        for (int h0 = 0; h0 < L::CPT; h0 += IL) {
+            frag_t B[IL][KC];
+            val_t nj[IL];
+            int SB[IL];
 #pragma unroll
-            for (int c = 0; c < KC; ++c) B[c] = reinterpret_cast<const frag_t *>(b)[(h * KC + c) * BFJ_WAVE + lane];
-            const val_t nj = reinterpret_cast<const val_t *>(b + L::FRAG)[h * 16 + (lane & 15)];
-            const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + h * 16 + (lane & 15)] : 127;
+            for (int u = 0; u < IL; ++u) {
+#pragma unroll
+                for (int c = 0; c < KC; ++c)
+                    B[u][c] = reinterpret_cast<const frag_t *>(b)[((h0 + u) * KC + c) * BFJ_WAVE + lane];
+                nj[u] = reinterpret_cast<const val_t *>(b + L::FRAG)[(h0 + u) * 16 + (lane & 15)];
+                SB[u] = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + (h0 + u) * 16 + (lane & 15)] : 127;
+            }
 
             // acc = g_i + xhat_i . xhat_j; the first MFMA reads g_i as its C operand, so
             // no VALU work precedes the MFMAs.  Z = acc - n_j is only formed for hot
             // tiles: acc < n_j is Z < 0 exactly, and max commutes with subtracting the
             // per-lane n_j.
-            acc_t acc[RB];
+            acc_t acc[IL][RB];
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][0], B[0], G[rb], SA[rb], SB);
+            for (int u = 0; u < IL; ++u)
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) acc[u][rb] = Tr::mma(A[rb][0], B[u][0], G[rb], SA[rb], SB[u]);
 #pragma unroll
             for (int c = 1; c < KC; ++c)
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][c], B[c], acc[rb], SA[rb], SB);
+                for (int u = 0; u < IL; ++u)
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) acc[u][rb] = Tr::mma(A[rb][c], B[u][c], acc[u][rb], SA[rb], SB[u]);
 
-            // Inputs are NaN-free (the pass rejects non-finite rows), so with
-            // -fno-honor-nans this is a tree of v_max3.
-            val_t m = acc[0][0];
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
+            for (int u = 0; u < IL; ++u) {
+                const int cb = cb0 + t * L::CPT + h0 + u;
+                // Inputs are NaN-free (the pass rejects non-finite rows), so with
+                // -fno-honor-nans this is a tree of v_max3.
+                val_t m = acc[u][0][0];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) m = fmax(m, acc[rb][r]);
+                for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) m = fmax(m, acc[u][rb][r]);
 
-            if (__builtin_expect(__ballot(m >= nj) != 0ull, 0)) {
-                // Rare: stream every pair with acc >= n_j (Z >= 0) as a candidate
-                // record; k_classify applies the per-eps tests to Z = acc - n_j.  In a
-                // symmetric sweep the tiles with cb < rb belong to another tile.
-                const int col = cb * 16 + (lane & 15);
-                const bool col_ok = col < s.n;
-                const bool sym = s.sym != 0;
-                acc_t *stage = reinterpret_cast<acc_t *>(lds + L::RING + wave * L::STAGE);
+                if (__builtin_expect(__ballot(m >= nj[u]) != 0ull, 0)) {
+                    // Rare: stream every pair with acc >= n_j (Z >= 0) as a candidate
+                    // record; k_classify applies the per-eps tests to Z = acc - n_j.  In a
+                    // symmetric sweep the tiles with cb < rb belong to another tile.
+                    const int col = cb * 16 + (lane & 15);
+                    const bool col_ok = col < s.n;
+                    const bool sym = s.sym != 0;
+                    acc_t *stage = reinterpret_cast<acc_t *>(lds + L::RING + wave * L::STAGE);
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) stage[rb * BFJ_WAVE + lane] = acc[rb];
+                    for (int rb = 0; rb < RB; ++rb) stage[rb * BFJ_WAVE + lane] = acc[u][rb];
 #pragma unroll 1
-                for (int rb = 0; rb < RB; ++rb) {
-                    const int blk = wrb0 + rb;
-                    if (sym && cb < blk) continue;
-                    cand_push_block<Tr>(a, cc, stage[rb * BFJ_WAVE + lane], nj, col, col_ok, blk * 16, sym,
-                                        (uint32_t)(s.row_off + blk * 16), lane);
+                    for (int rb = 0; rb < RB; ++rb) {
+                        const int blk = wrb0 + rb;
+                        if (sym && cb < blk) continue;
+                        cand_push_block<Tr>(a, cc, stage[rb * BFJ_WAVE + lane], nj[u], col, col_ok, blk * 16, sym,
+                                            (uint32_t)(s.row_off + blk * 16), lane);
+                    }
                 }
             }
         }
