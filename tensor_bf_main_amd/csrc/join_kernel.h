@@ -379,7 +379,12 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 // it is empty.  All waves of a block stay together (the candidate tiles are
 // shared), so the queue is read by one thread and broadcast through LDS.
 template <class Tr, int KC, int R>
-__global__ __launch_bounds__(BFJ_JBLOCK) void k_join(JoinArgs a)
+#ifdef BFJ_WAVES_PER_EU
+#define BFJ_JOIN_ATTR __attribute__((amdgpu_waves_per_eu(BFJ_WAVES_PER_EU, BFJ_WAVES_PER_EU)))
+#else
+#define BFJ_JOIN_ATTR
+#endif
+__global__ __launch_bounds__(BFJ_JBLOCK) BFJ_JOIN_ATTR void k_join(JoinArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     __shared__ uint32_t item_s;
