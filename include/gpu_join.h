@@ -7,6 +7,7 @@
 #define BFJ_GPU_JOIN_H
 
 #include <stdint.h>
+#include <vector>
 
 #include "bfjoin.h"
 #include "params.h"
@@ -38,6 +39,16 @@ void GPUJoinMainBruteForceNvidia(unsigned int searchMode, unsigned int device, I
 int GPUJoinMainBruteForceEx(unsigned int searchMode, unsigned int device, const double *dataset, unsigned int n,
                             unsigned int dim, unsigned int computeDim, double epsilon, uint64_t *totalNeighbors,
                             int64_t *counts, bfj_stats *stats);
+
+/*
+ * Result-set variant (no counterpart in the reference, which only counts): every
+ * unordered pair i < j within epsilon, appended to *pairs as (i, j), in no
+ * particular order.  *totalNeighbors gets the reference's ordered, self-inclusive
+ * total (n + 2 * pairs).  Returns a bfj_status.
+ */
+int GPUJoinMainBruteForcePairs(unsigned int searchMode, unsigned int device, const double *dataset, unsigned int n,
+                               unsigned int dim, unsigned int computeDim, double epsilon,
+                               std::vector<uint32_t> *pairs, uint64_t *totalNeighbors, bfj_stats *stats);
 
 /* searchMode -> bfj_dtype, or -1 for modes that are errors (5, 6, unknown). */
 int searchModeDtype(unsigned int searchMode);

@@ -419,9 +419,10 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     }
     RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
     if (ctx->chunk_capacity == 0) {
-        // Start at 32 candidates per row (and at least 4 chunks per join block of the
-        // GPU); a pass that needs more is repeated with the size it reported.
-        const uint64_t want = std::max<uint64_t>((uint64_t)p.rows_total * 32 / BFJ_CHUNK, 4096);
+        // Start at 96 candidates per row (1.5 GiB at a million rows; about 100 neighbours
+        // per point fit) and at least 4 chunks per join wavefront of the GPU; a pass
+        // that needs more is repeated with the size it reported.
+        const uint64_t want = std::max<uint64_t>((uint64_t)p.rows_total * 96 / BFJ_CHUNK, 16384);
         ctx->chunk_capacity = (uint32_t)std::min<uint64_t>(want, 1u << 24);
     }
     RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));

@@ -2,6 +2,7 @@
 // the C ABI.  No HIP header is included: everything device-side is behind bfjoin.h.
 #include "gpu_join.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -70,6 +71,38 @@ int GPUJoinMainBruteForceEx(unsigned int searchMode, unsigned int device, const 
         std::fprintf(stderr, "[GPU] ~ Error: %s\n", bfj_last_error());
         return rc;
     }
+    if (totalNeighbors) *totalNeighbors = total;
+    return BFJ_OK;
+}
+
+int GPUJoinMainBruteForcePairs(unsigned int searchMode, unsigned int device, const double *dataset, unsigned int n,
+                               unsigned int dim, unsigned int computeDim, double epsilon,
+                               std::vector<uint32_t> *pairs, uint64_t *totalNeighbors, bfj_stats *stats)
+{
+    const int dtype = searchModeDtype(searchMode);
+    if (dtype < 0) {
+        std::fprintf(stderr, "[GPU] ~ Error: Unknown search mode\n");
+        return BFJ_ERR_INVALID;
+    }
+    bfj_ctx *ctx = context_for(device);
+    if (!ctx) return BFJ_ERR_DEVICE;
+    // Start at 16 pairs per point; the library reports the size it needs when that
+    // is not enough, and the call is repeated once with it.
+    uint64_t cap = std::max<uint64_t>(16ull * n, 1024), need = 0, total = 0;
+    int rc = BFJ_OK;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        pairs->assign(2 * cap, 0u);
+        rc = bfj_self_join_pairs(ctx, dataset, (int64_t)n, (int32_t)dim, (int64_t)computeDim, epsilon, dtype,
+                                 pairs->data(), cap, &need, &total, stats);
+        if (rc != BFJ_ERR_CAPACITY) break;
+        cap = need;
+    }
+    if (rc != BFJ_OK) {
+        std::fprintf(stderr, "[GPU] ~ Error: %s\n", bfj_last_error());
+        pairs->clear();
+        return rc;
+    }
+    pairs->resize(2 * need);
     if (totalNeighbors) *totalNeighbors = total;
     return BFJ_OK;
 }

@@ -1,7 +1,7 @@
 // main.cpp -- the reference's CLI (main.cu:16-128) on top of gpu_join.h.
 //
 //   bfjoin_main <file> <eps> <searchMode> [device] [--dim D] [--compute-dim C]
-//               [--binary] [--counts out.txt] [--log file] [--parse-only]
+//               [--binary] [--counts out.txt] [--pairs out.txt] [--log file] [--parse-only]
 //
 // Same positional arguments, console lines and CSV result log as the reference;
 // the options give the row length at run time (the reference fixes it at compile
@@ -40,7 +40,7 @@ int main(int argc, char *argv[])
     if (argc < 4) {
         std::cerr << "usage: " << argv[0]
                   << " <file> <eps> <searchMode> [device] [--dim D] [--compute-dim C] [--binary]"
-                     " [--counts out.txt] [--log file]\n";
+                     " [--counts out.txt] [--pairs out.txt] [--log file]\n";
         return 1;
     }
     const char *filename = argv[FILENAME_ARG];
@@ -49,7 +49,7 @@ int main(int argc, char *argv[])
     unsigned int device = 0;
     int dim = INPUT_DATA_DIM, computeDim = -1;
     bool binary = false, parse_only = false;
-    std::string counts_path, log_path = "tensor_brute-force.txt";
+    std::string counts_path, pairs_path, log_path = "tensor_brute-force.txt";
     int pos = 0;
     for (int a = 4; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--dim") && a + 1 < argc) dim = std::atoi(argv[++a]);
@@ -57,6 +57,7 @@ int main(int argc, char *argv[])
         else if (!std::strcmp(argv[a], "--binary")) binary = true;
         else if (!std::strcmp(argv[a], "--parse-only")) parse_only = true;
         else if (!std::strcmp(argv[a], "--counts") && a + 1 < argc) counts_path = argv[++a];
+        else if (!std::strcmp(argv[a], "--pairs") && a + 1 < argc) pairs_path = argv[++a];
         else if (!std::strcmp(argv[a], "--log") && a + 1 < argc) log_path = argv[++a];
         else if (pos++ == 0) device = (unsigned)std::atoi(argv[a]); // DEVICE_ARG
         else {
@@ -138,6 +139,17 @@ int main(int argc, char *argv[])
     if (!counts_path.empty()) {
         std::ofstream cf(counts_path);
         for (int64_t c : counts) cf << c << '\n';
+    }
+    if (!pairs_path.empty()) {
+        // The result set itself: "i j" per line, i < j, in no particular order.
+        std::vector<uint32_t> pairs;
+        uint64_t total = 0;
+        if (GPUJoinMainBruteForcePairs(searchMode, device, database.data(), nbQueryPoints, (unsigned)dim,
+                                       (unsigned)computeDim, epsilon, &pairs, &total, nullptr) != BFJ_OK)
+            return 1;
+        std::ofstream pf(pairs_path);
+        for (size_t k = 0; k + 1 < pairs.size(); k += 2) pf << pairs[k] << ' ' << pairs[k + 1] << '\n';
+        std::cout << "[Main | Result] ~ Pairs written: " << pairs.size() / 2 << '\n';
     }
 
     // main.cu:115-127: append one CSV row, header first if the file is empty.  The

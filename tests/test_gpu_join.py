@@ -231,10 +231,15 @@ def test_cli_drop_in(tmp_path, mode, dtype):
     k = gold["eps"].index(0.05)
     log = tmp_path / "log.csv"
     counts = tmp_path / "counts.txt"
+    pairs = tmp_path / "pairs.txt"
     r = subprocess.run([bfj.CLI_PATH, os.path.join(GOLDEN, "test128.txt"), "0.05", str(mode), "0", "--dim", "18",
-                        "--log", str(log), "--counts", str(counts)], capture_output=True, text=True, timeout=120)
+                        "--log", str(log), "--counts", str(counts), "--pairs", str(pairs)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert f"[Main | Result] ~ Total result set size: {gold['totals'][k]}" in r.stdout
+    # The result set: (total - n) / 2 unordered pairs, i < j, no repeats.
+    got = [tuple(int(v) for v in ln.split()) for ln in pairs.read_text().splitlines()]
+    assert len(got) == len(set(got)) == (gold["totals"][k] - 128) // 2 and all(i < j for i, j in got)
     rows = log.read_text().splitlines()
     assert rows[0].startswith("Dataset, epsilon, searchMode, executionTime, totalNeighbors, inputDim, computeDim")
     f = [v.strip() for v in rows[1].split(",")]
