@@ -131,14 +131,14 @@ static int lds_for(int dtype, int kc, int rows, int n_eps)
     return 0;
 }
 
-// Default tile: the most resident blocks per CU (counted up to 4: latency hiding
+// Default tile: the most resident blocks per CU (counted up to 3: latency hiding
 // across blocks matters more than strip width), then the widest strip.
 static int heuristic_rows(int dtype, int kc, int n_eps)
 {
     int best = 0, best_occ = -1;
     for (int r : kRowOptions)
         if (bfj_join_has(dtype, kc, r)) {
-            const int occ = std::min(4, bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps)));
+            const int occ = std::min(3, bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps)));
             if (occ >= best_occ) {
                 best = r;
                 best_occ = occ;

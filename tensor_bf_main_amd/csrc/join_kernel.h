@@ -87,8 +87,8 @@ __device__ inline int bfj_lane() { return __lane_id(); }
 #ifndef BFJ_SLOT_BYTES
 #define BFJ_SLOT_BYTES 8192 // candidate fragment bytes per ring slot
 #endif
-#ifndef BFJ_INTERLEAVE
-#define BFJ_INTERLEAVE 1 // candidate blocks whose MFMA chains are interleaved
+#ifndef BFJ_DIRECT_B
+#define BFJ_DIRECT_B 0 // 1: candidate fragments read straight into VGPRs, no LDS ring
 #endif
 #ifndef BFJ_RING_BYTES
 #define BFJ_RING_BYTES (24 * 1024) // target LDS bytes of the candidate ring
@@ -126,7 +126,8 @@ struct JoinLds {
     // Per-wavefront staging of a hot tile's accumulators (lane-private, so the
     // candidate push can loop over row blocks without holding them in VGPRs).
     static constexpr int STAGE = RB * BFJ_WAVE * (int)sizeof(typename Tr::acc_t);
-    __host__ __device__ static constexpr int total(int) { return RING + BFJ_JWAVES * STAGE; }
+    static constexpr int STAGE_OFF = BFJ_DIRECT_B ? 0 : RING; // no ring when fragments go to VGPRs
+    __host__ __device__ static constexpr int total(int) { return STAGE_OFF + BFJ_JWAVES * STAGE; }
 };
 
 // Per-wavefront cursor into the candidate stream: chunks of BFJ_CHUNK records
@@ -275,6 +276,80 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         else __builtin_amdgcn_s_waitcnt((N2 & 15) | ((N2 >> 4) << 14) | (7 << 4) | (15 << 8));
     };
 
+    // One candidate block cb against this wavefront's rows.
+    auto block = [&](int cb, const frag_t (&B)[KC], val_t nj, int SB) {
+        // acc = g_i + xhat_i . xhat_j; the first MFMA reads g_i as its C operand, so no
+        // VALU work precedes the MFMAs.  Z = acc - n_j is only formed for hot tiles:
+        // acc < n_j is Z < 0 exactly, and max commutes with subtracting the per-lane n_j.
+        acc_t acc[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][0], B[0], G[rb], SA[rb], SB);
+#pragma unroll
+        for (int c = 1; c < KC; ++c)
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][c], B[c], acc[rb], SA[rb], SB);
+
+        // Inputs are NaN-free (the pass rejects non-finite rows), so with -fno-honor-nans
+        // these are v_max3 trees: one per row block, then one over the blocks.
+        val_t mrb[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) mrb[rb] = fmax(fmax(acc[rb][0], acc[rb][1]), fmax(acc[rb][2], acc[rb][3]));
+        val_t m = mrb[0];
+#pragma unroll
+        for (int rb = 1; rb < RB; ++rb) m = fmax(m, mrb[rb]);
+
+        if (__builtin_expect(__ballot(m >= nj) != 0ull, 0)) {
+            // Rare: stream every pair with acc >= n_j (Z >= 0) as a candidate record;
+            // k_classify applies the per-eps tests to Z = acc - n_j.  Only the row blocks
+            // that hold such a pair are visited; in a symmetric sweep the tiles with
+            // cb < rb belong to another tile.  The accumulators go through a
+            // wavefront-private LDS area so that the loop need not keep them all live.
+            const bool sym = s.sym != 0;
+            uint32_t hot = 0;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+                if (__ballot(mrb[rb] >= nj) != 0ull && !(sym && cb < wrb0 + rb)) hot |= 1u << rb;
+            const int col = cb * 16 + (lane & 15);
+            const bool col_ok = col < s.n;
+            acc_t *stage = reinterpret_cast<acc_t *>(lds + L::STAGE_OFF + wave * L::STAGE);
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+                if ((hot >> rb) & 1u) stage[rb * BFJ_WAVE + lane] = acc[rb];
+            while (hot) {
+                const int rb = __builtin_ctz(hot);
+                hot &= hot - 1u;
+                const int blk = wrb0 + rb;
+                cand_push_block<Tr>(a, cc, stage[rb * BFJ_WAVE + lane], nj, col, col_ok, blk * 16, sym,
+                                    (uint32_t)(s.row_off + blk * 16), lane);
+            }
+        }
+    };
+
+    if (BFJ_DIRECT_B) {
+        // Candidate fragments straight from global memory (L2-resident panel) into
+        // VGPRs, one block ahead; no ring, no barrier.
+        frag_t Bn[KC];
+        val_t njn = 0;
+        int SBn = 127;
+        auto fetch = [&](int cb) {
+#pragma unroll
+            for (int c = 0; c < KC; ++c) Bn[c] = P[((int64_t)cb * KC + c) * BFJ_WAVE + lane];
+            njn = nq[cb * 16 + (lane & 15)];
+            if (Tr::dtype == BFJ_FP8) SBn = (int)sc[cb * 16 + (lane & 15)];
+        };
+        fetch(cb0);
+        for (int cb = cb0; cb < cb1; ++cb) {
+            frag_t B[KC];
+#pragma unroll
+            for (int c = 0; c < KC; ++c) B[c] = Bn[c];
+            const val_t nj = njn;
+            const int SB = SBn;
+            if (cb + 1 < cb1) fetch(cb + 1);
+            if (any_rows) block(cb, B, nj, SB);
+        }
+        return;
+    }
+
     // Drain the strip's own loads here, so that inside the loop the only
     // outstanding vector-memory loads are the ring's.
     __builtin_amdgcn_s_waitcnt(0);
@@ -299,69 +374,14 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         issue(t + L::NB - 1, rcur == 0 ? L::NB - 1 : rcur - 1); // into the buffer of slot t-1
         if (!any_rows) return;
         const unsigned char *b = lds + rcur * L::BUF;
-        // IL blocks at a time: their MFMA chains interleave (more independent
-        // accumulators in flight) before the max trees read them.
-        constexpr int IL = L::CPT < BFJ_INTERLEAVE ? L::CPT : BFJ_INTERLEAVE;
 #pragma unroll
-        for (int h0 = 0; h0 < L::CPT; h0 += IL) {
-            frag_t B[IL][KC];
-            val_t nj[IL];
-            int SB[IL];
+        for (int h = 0; h < L::CPT; ++h) {
+            frag_t B[KC];
 #pragma unroll
-            for (int u = 0; u < IL; ++u) {
-#pragma unroll
-                for (int c = 0; c < KC; ++c)
-                    B[u][c] = reinterpret_cast<const frag_t *>(b)[((h0 + u) * KC + c) * BFJ_WAVE + lane];
-                nj[u] = reinterpret_cast<const val_t *>(b + L::FRAG)[(h0 + u) * 16 + (lane & 15)];
-                SB[u] = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + (h0 + u) * 16 + (lane & 15)] : 127;
-            }
-
-            // acc = g_i + xhat_i . xhat_j; the first MFMA reads g_i as its C operand, so
-            // no VALU work precedes the MFMAs.  Z = acc - n_j is only formed for hot
-            // tiles: acc < n_j is Z < 0 exactly, and max commutes with subtracting the
-            // per-lane n_j.
-            acc_t acc[IL][RB];
-#pragma unroll
-            for (int u = 0; u < IL; ++u)
-#pragma unroll
-                for (int rb = 0; rb < RB; ++rb) acc[u][rb] = Tr::mma(A[rb][0], B[u][0], G[rb], SA[rb], SB[u]);
-#pragma unroll
-            for (int c = 1; c < KC; ++c)
-#pragma unroll
-                for (int u = 0; u < IL; ++u)
-#pragma unroll
-                    for (int rb = 0; rb < RB; ++rb) acc[u][rb] = Tr::mma(A[rb][c], B[u][c], acc[u][rb], SA[rb], SB[u]);
-
-#pragma unroll
-            for (int u = 0; u < IL; ++u) {
-                const int cb = cb0 + t * L::CPT + h0 + u;
-                // Inputs are NaN-free (the pass rejects non-finite rows), so with
-                // -fno-honor-nans this is a tree of v_max3.
-                val_t m = acc[u][0][0];
-#pragma unroll
-                for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) m = fmax(m, acc[u][rb][r]);
-
-                if (__builtin_expect(__ballot(m >= nj[u]) != 0ull, 0)) {
-                    // Rare: stream every pair with acc >= n_j (Z >= 0) as a candidate
-                    // record; k_classify applies the per-eps tests to Z = acc - n_j.  In a
-                    // symmetric sweep the tiles with cb < rb belong to another tile.
-                    const int col = cb * 16 + (lane & 15);
-                    const bool col_ok = col < s.n;
-                    const bool sym = s.sym != 0;
-                    acc_t *stage = reinterpret_cast<acc_t *>(lds + L::RING + wave * L::STAGE);
-#pragma unroll
-                    for (int rb = 0; rb < RB; ++rb) stage[rb * BFJ_WAVE + lane] = acc[u][rb];
-#pragma unroll 1
-                    for (int rb = 0; rb < RB; ++rb) {
-                        const int blk = wrb0 + rb;
-                        if (sym && cb < blk) continue;
-                        cand_push_block<Tr>(a, cc, stage[rb * BFJ_WAVE + lane], nj[u], col, col_ok, blk * 16, sym,
-                                            (uint32_t)(s.row_off + blk * 16), lane);
-                    }
-                }
-            }
+            for (int c = 0; c < KC; ++c) B[c] = reinterpret_cast<const frag_t *>(b)[(h * KC + c) * BFJ_WAVE + lane];
+            const val_t nj = reinterpret_cast<const val_t *>(b + L::FRAG)[h * 16 + (lane & 15)];
+            const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + h * 16 + (lane & 15)] : 127;
+            block(cb0 + t * L::CPT + h, B, nj, SB);
         }
     };
     // Unrolled by NB so that every ring address is a constant.
