@@ -24,13 +24,13 @@
 //     order at unit roundoff u (2^-24 for FP32 accumulation, 2^-53 for FP64) on
 //     partial sums within |g_i| + sum |p_k| <= |g_i| + sqrt(N_i N_max), plus the
 //     final subtraction and the FP32 rounding of n_j.
-//   So S_hat = (2 g_i + N_i) - 2 Z satisfies |S_hat - S_ij| <= E_i, and for a row
-//   k of the pair (k = i or j) and eps e:
-//     certainly within eps   if S_hat + E_i <= L_ke = (sqrt(T_e/(1+rho)) - r_k)^2
-//     certainly outside      if S_hat - E_i >  H_ke = (sqrt(T_e/(1-rho)) + r_k)^2
-//   with r_k = q_k + q_max.  g_i = 0.5 (H* + E_i - N_i) rounded up, H* the H of
-//   r = 2 q_max (>= every H_ke), makes Z < 0 certainly outside for every eps and
-//   both rows: one Z per unordered pair decides for i and for j.
+//   So S_hat = (2 g_i + N_i) - 2 Z satisfies |S_hat - S_ij| <= E_i, and for eps e:
+//     certainly within eps   if S_hat + E_i <= L_e = (sqrt(T_e/(1+rho)) - r)^2
+//     certainly outside      if S_hat - E_i >  H_e = (sqrt(T_e/(1-rho)) + r)^2
+//   with r = q_i + q_j (one bound for both rows of the pair).  g_i = 0.5 (Hmax_i +
+//   E_i - N_i) rounded up, Hmax_i the largest H of r = q_i + q_max (>= H(r_ij) for
+//   every j), makes Z < 0 certainly outside for every eps and both rows: one Z per
+//   unordered pair decides for i and for j.
 #include "bfj_internal.h"
 #include <math.h>
 #include <float.h>
@@ -349,9 +349,10 @@ __global__ void k_thresholds(PassArgs a)
         const double Dmax2 = dmax * dmax * (1.0 + rho) * (1.0 + 0x1p-40);
         double Hc[BFJ_MAXEPS], Lc[BFJ_MAXEPS];
         uint32_t all_mask = 0;
-        // H* bounds H of every row of the dataset (r <= 2 q_max), so that Z < 0 is a
-        // certain reject for the row of the tile and for the candidate row alike.
-        double Hstar = -INFINITY;
+        // Hmax: the largest banded H of this row (r = q_i + q_max).  Every pair (i, j)
+        // is decided with its own r_ij = q_i + q_j <= q_i + q_max, so H(r_ij) <= Hmax
+        // and Z < 0 is a certain reject of the pair, for both of its rows.
+        double Hmax = -INFINITY;
         for (int e = 0; e < s.n_eps; ++e) {
             const double T = s.T[e] / sigma2; // kernel space (exact power-of-two scaling)
             Hc[e] = Lc[e] = -INFINITY;
@@ -365,14 +366,13 @@ __global__ void k_thresholds(PassArgs a)
                 Lc[e] = slo > r ? (slo - r) * (slo - r) * (1.0 - 0x1p-49) : -INFINITY;
                 Hc[e] = (shi + r) * (shi + r) * (1.0 + 0x1p-49);
                 if (query) halfband = fmax(halfband, 0.5 * (Hc[e] - (Lc[e] > 0.0 ? Lc[e] : 0.0)) * sigma2);
-                const double rs = 2.0 * qmax;
-                Hstar = fmax(Hstar, (shi + rs) * (shi + rs) * (1.0 + 0x1p-49));
+                Hmax = fmax(Hmax, Hc[e]);
             }
         }
         if (i == 0 && all_mask) atomicOr(&a.scal[si].all_mask, (unsigned long long)all_mask);
         double g; // FP64 value of the per-row initial term actually stored
         double E = 0.0;
-        if (!query || Hstar == -INFINITY) {
+        if (!query || Hmax == -INFINITY) {
             g = fp64 ? -1e300 : -1e30;
         } else {
             // Accumulation bound; |g| is bounded from the previous estimate.  The MFMA
@@ -381,15 +381,15 @@ __global__ void k_thresholds(PassArgs a)
             const double Kp = 2.0 * K + 8.0;
             const double coef = 2.0 * (1.01 * Kp + 1.0) * u;
             const double cross = sqrt(Ni * nmax);
-            double gabs = 0.5 * fabs(Hstar - Ni);
+            double gabs = 0.5 * fabs(Hmax - Ni);
             for (int it = 0; it < 2; ++it) {
                 E = coef * (gabs + cross) + 4.0 * u * nmax + 4.0 * (K + 4) * u64 * (Ni + nmax + gabs) + K * 0x1p-120;
-                gabs = 0.5 * fabs(Hstar - Ni) + E;
+                gabs = 0.5 * fabs(Hmax - Ni) + E;
             }
             E *= 1.01;
-            // Rounded up whatever its sign: 2 g + N_i - E >= H* makes Z < 0 a certain
-            // reject (S >= S_hat - E > H*) for every banded eps and both rows.
-            const double x = 0.5 * (Hstar + E - Ni);
+            // Rounded up whatever its sign: 2 g + N_i - E >= Hmax makes Z < 0 a certain
+            // reject (S >= S_hat - E > Hmax >= H(r_ij)) for every banded eps.
+            const double x = 0.5 * (Hmax + E - Ni);
             const double gv = x + fabs(x) * 0x1p-50 + 0x1p-1000;
             g = fp64 ? gv : (double)f32_up(gv);
         }
