@@ -182,7 +182,7 @@ struct bfj_ctx {
     int n_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
-    DevBuf pack, nq, gq, he, vin, row_set, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
+    DevBuf pack, nq, gq, he, row_set, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
         outbuf, cand, chunk_fill, pairs;
     uint32_t band_capacity = 0;
     uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
@@ -408,7 +408,6 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->nq, (nrows + 1024) * vb));
     RC(ensure(ctx->gq, nrows * vb));
     RC(ensure(ctx->he, nrows * 16));
-    RC(ensure(ctx->vin, nrows * vb));
     RC(ensure(ctx->row_set, nrows * 2));
     RC(ensure(ctx->qerr, nrows * 8));
     RC(ensure(ctx->nrm, nrows * 8));
@@ -474,7 +473,6 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.nq = ctx->nq.p;
     a.gq = ctx->gq.p;
     a.he = (double2 *)ctx->he.p;
-    a.vin = ctx->vin.p;
     a.row_set = (uint16_t *)ctx->row_set.p;
     a.qerr = (double *)ctx->qerr.p;
     a.nrm = (double *)ctx->nrm.p;
@@ -525,9 +523,6 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
         j.chunk_fill = (uint32_t *)ctx->chunk_fill.p;
         j.chunk_count = (uint32_t *)ctx->counters.p + 1;
         j.chunk_capacity = ctx->chunk_capacity;
-        j.scal = a.scal;
-        j.vin = a.vin;
-        j.cnt = a.cnt;
         JoinLaunch L;
         L.dtype = p.dtype;
         L.kc = g.kc;
@@ -708,7 +703,7 @@ int bfj_destroy(bfj_ctx *c)
     if (!c) return BFJ_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->he, &c->vin, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
+    DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
                       &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
                       &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs};
     for (DevBuf *b : bufs) release(*b);

@@ -68,8 +68,6 @@ struct SetScalars {
     unsigned long long nonfinite;     // non-zero if some row holds NaN or inf
     unsigned long long all_mask;      // eps whose T exceeds every possible distance: all pairs hit
     unsigned long long band_mask;     // eps decided by the join + refine (neither none nor all)
-    int32_t band_first;               // first banded eps (ascending order), -1 if none
-    int32_t pad_;
     double slo[BFJ_MAXEPS];           // sqrt(T/(1+rho)) rounded down (kernel space)
     double shi[BFJ_MAXEPS];           // sqrt(T/(1-rho)) rounded up
     unsigned long long total[BFJ_MAXEPS]; // totals, self included
@@ -116,9 +114,6 @@ struct JoinArgs {
     uint32_t *chunk_fill;    // records written into each chunk
     uint32_t *chunk_count;   // chunks handed out (may exceed chunk_capacity: overflow)
     uint32_t chunk_capacity;
-    const SetScalars *scal;  // per dataset (band_first)
-    const void *vin;         // per row: Z at or above this is certainly within the first banded eps
-    uint32_t *cnt;           // per-bucket pair counts (pairs the join decides itself)
 };
 
 // Host-side launchers (one translation unit per dtype; see join_*.hip).
@@ -155,7 +150,6 @@ struct PassArgs {
     unsigned char *pack;
     void *nq, *gq;
     double2 *he;               // per row
-    void *vin;                 // per row, float or double: certain-in threshold on Z (join side)
     uint16_t *row_set;         // per row: index of its dataset
     double *qerr, *nrm;        // per row
     uint8_t *scales;

@@ -126,10 +126,8 @@ struct JoinLds {
     // Per-wavefront staging of a hot tile's accumulators (lane-private, so the
     // candidate push can loop over row blocks without holding them in VGPRs).
     static constexpr int STAGE = RB * BFJ_WAVE * (int)sizeof(typename Tr::acc_t);
-    // Then the wavefront's certain-in thresholds and certain-in pair counts, per row.
-    static constexpr int VIN = STAGE, RCNT = VIN + R * (int)sizeof(val_t), WAVE_AREA = RCNT + R * 4;
     static constexpr int STAGE_OFF = BFJ_DIRECT_B ? 0 : RING; // no ring when fragments go to VGPRs
-    __host__ __device__ static constexpr int total(int) { return STAGE_OFF + BFJ_JWAVES * WAVE_AREA; }
+    __host__ __device__ static constexpr int total(int) { return STAGE_OFF + BFJ_JWAVES * STAGE; }
 };
 
 // Per-wavefront cursor into the candidate stream: chunks of BFJ_CHUNK records
@@ -150,49 +148,19 @@ __device__ inline void cand_close(const JoinArgs &a, const CandCursor &cc, int l
 // plain 16-byte stores.  In a symmetric sweep only pairs above the diagonal
 // (col > row) are kept: the tile that holds (j, i) is dropped by the caller and
 // self pairs are counted in k_finalize.
-//
-// Counting (vin != NULL): a pair with Z >= vin (the row's certain-in threshold for
-// the first banded eps) is counted here instead: for its row in the wavefront's LDS
-// counters (lrow, this block's 16 rows), and in a symmetric sweep for its column with
-// one global atomic per column of the block (gcol = cnt bucket of the dataset).
 template <class Tr>
 __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &cc, const typename Tr::acc_t &v,
                                                 typename Tr::val_t nj, int col, bool col_ok, int li0, bool sym,
-                                                uint32_t row0, int lane, const typename Tr::val_t *vin,
-                                                uint32_t *lrow, uint32_t *gcol)
+                                                uint32_t row0, int lane)
 {
-    typedef typename Tr::val_t val_t;
-    unsigned long long bal[4], bin[4];
+    unsigned long long bal[4];
     uint32_t cnt[4], nrec = 0;
-    bool any_in = false;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int li = li0 + Tr::out_row(lane, r);
-        const bool ok = v[r] >= nj && col_ok && (!sym || col > li);
-        const bool in = vin && ok && (val_t)(v[r] - nj) >= vin[Tr::out_row(lane, r)];
-        bin[r] = __ballot(in);
-        bal[r] = __ballot(ok && !in);
-        any_in |= bin[r] != 0ull;
+        bal[r] = __ballot(v[r] >= nj && col_ok && (!sym || col > li));
         cnt[r] = (uint32_t)__popcll(bal[r]);
         nrec += cnt[r];
-    }
-    if (any_in) {
-        // Rows: lane k < 16 counts block row k (register r, lane group g).
-        if (lane < 16) {
-            const int r = Tr::dtype == BFJ_FP64 ? (lane >> 2) : (lane & 3);
-            const int g = Tr::dtype == BFJ_FP64 ? (lane & 3) : (lane >> 2);
-            const unsigned long long b = r == 0 ? bin[0] : (r == 1 ? bin[1] : (r == 2 ? bin[2] : bin[3]));
-            const uint32_t c = (uint32_t)__popcll((b >> (16 * g)) & 0xffffull);
-            if (c) atomicAdd(&lrow[lane], c);
-        }
-        // Columns (symmetric sweep): lane c < 16 sums column c over the 4 registers
-        // and the 4 lane groups.
-        if (sym && lane < 16) {
-            uint32_t c = 0;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) c += (uint32_t)__popcll((bin[r] >> lane) & 0x0001000100010001ull);
-            if (c) atomicAdd(&gcol[col], c);
-        }
     }
     if (nrec == 0) return;
     if (cc.fill + nrec > BFJ_CHUNK) { // nrec <= 64 fits a fresh chunk
@@ -224,7 +192,8 @@ __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &c
 
 template <class Tr, int KC, int R>
 __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, int rb0, int cb0, int cb1,
-                                           int bucket0, unsigned char *lds, CandCursor &cc)
+                                           unsigned char *lds,
+                                           CandCursor &cc)
 {
     typedef typename Tr::frag_t frag_t;
     typedef typename Tr::acc_t acc_t;
@@ -258,20 +227,6 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             G[rb][r] = ok ? gq[row] : (val_t)-1e30;
         }
         SA[rb] = (Tr::dtype == BFJ_FP8 && ok) ? (int)sc[blk * 16 + (lane & 15)] : 127;
-    }
-    // The wavefront's certain-in thresholds into LDS (read by the slow path only) and
-    // its row counters cleared.  bucket0 < 0: no banded eps, nothing is counted.
-    unsigned char *warea = lds + L::STAGE_OFF + wave * L::WAVE_AREA;
-    val_t *vin_l = reinterpret_cast<val_t *>(warea + L::VIN);
-    uint32_t *rcnt = reinterpret_cast<uint32_t *>(warea + L::RCNT);
-    const bool counting = bucket0 >= 0 && a.vin != nullptr;
-    const val_t *vin_g = reinterpret_cast<const val_t *>(a.vin) + s.row_off;
-    uint32_t *gcnt = a.cnt + s.cnt_off + (int64_t)(bucket0 < 0 ? 0 : bucket0) * s.npad;
-#pragma unroll
-    for (int k = lane; k < R; k += BFJ_WAVE) {
-        const int64_t row = (int64_t)wrb0 * 16 + k;
-        vin_l[k] = (counting && row < s.npad) ? vin_g[row] : (val_t)INFINITY;
-        rcnt[k] = 0;
     }
 
     // Candidate tiles stream through an NB-deep LDS ring filled by direct-to-LDS
@@ -356,7 +311,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
                 if (__ballot(mrb[rb] >= nj) != 0ull && !(sym && cb < wrb0 + rb)) hot |= 1u << rb;
             const int col = cb * 16 + (lane & 15);
             const bool col_ok = col < s.n;
-            acc_t *stage = reinterpret_cast<acc_t *>(warea);
+            acc_t *stage = reinterpret_cast<acc_t *>(lds + L::STAGE_OFF + wave * L::STAGE);
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb)
                 if ((hot >> rb) & 1u) stage[rb * BFJ_WAVE + lane] = acc[rb];
@@ -365,19 +320,8 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
                 hot &= hot - 1u;
                 const int blk = wrb0 + rb;
                 cand_push_block<Tr>(a, cc, stage[rb * BFJ_WAVE + lane], nj, col, col_ok, blk * 16, sym,
-                                    (uint32_t)(s.row_off + blk * 16), lane, counting ? vin_l + rb * 16 : nullptr,
-                                    rcnt + rb * 16, gcnt);
+                                    (uint32_t)(s.row_off + blk * 16), lane);
             }
-        }
-    };
-
-    // Row counts of the strip to global memory: one atomic per row that has pairs.
-    auto flush_rows = [&]() {
-        if (!counting) return;
-#pragma unroll
-        for (int k = lane; k < R; k += BFJ_WAVE) {
-            const uint32_t c = rcnt[k];
-            if (c) atomicAdd(&gcnt[(int64_t)wrb0 * 16 + k], c);
         }
     };
 
@@ -403,7 +347,6 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             if (cb + 1 < cb1) fetch(cb + 1);
             if (any_rows) block(cb, B, nj, SB);
         }
-        flush_rows();
         return;
     }
 
@@ -447,7 +390,6 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         for (int j = 0; j < L::NB; ++j)
             if (t0 + j < nt) step(t0 + j, j);
     }
-    flush_rows();
     // The prefetched tail loads must land before the next strip reuses the ring.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -478,8 +420,7 @@ __global__ __launch_bounds__(BFJ_JBLOCK) BFJ_JOIN_ATTR void k_join(JoinArgs a)
         if (item >= (uint32_t)a.n_items) break;
         const WorkItem w = a.items[item];
         const SetDesc s = a.sets[w.set];
-        const int bucket0 = __builtin_amdgcn_readfirstlane(a.scal[w.set].band_first);
-        join_strip<Tr, KC, R>(a, s, w.rb0, w.cb0, w.cb1, bucket0, lds_raw, cc);
+        join_strip<Tr, KC, R>(a, s, w.rb0, w.cb0, w.cb1, lds_raw, cc);
     }
     cand_close(a, cc, lane);
 }

@@ -395,28 +395,9 @@ __global__ void k_thresholds(PassArgs a)
         }
         if (fp64) ((double *)a.gq)[row] = g;
         else ((float *)a.gq)[row] = (float)g;
-        const double hh = 2.0 * g + Ni;
-        a.he[row] = make_double2(hh, E);
+        a.he[row] = make_double2(2.0 * g + Ni, E);
         a.row_set[row] = (uint16_t)si;
-        // Join-side certain-in threshold for the first banded eps b0 (ascending order):
-        // Z >= vin gives S <= S_hat + E_i + (rounding) <= L_b0 with r = q_i + q_max >=
-        // q_i + q_j, so the pair is within every banded eps for both of its rows and
-        // the join counts it itself.  +inf (never) for rows that are not queries, when
-        // L is empty, and in pair-list mode (every pair must become a record).
-        int b0 = -1;
-        for (int e = 0; e < s.n_eps && b0 < 0; ++e)
-            if (Hc[e] != -INFINITY) b0 = e;
-        double vin = INFINITY;
-        if (query && b0 >= 0 && Lc[b0] != -INFINITY && !a.emit_pairs) {
-            const double L = Lc[b0];
-            double x = 0.5 * (hh + E - L);
-            x += fabs(x) * 0x1p-50 + (fabs(hh) + E + fabs(L)) * 0x1p-48;
-            vin = x;
-        }
-        if (fp64) ((double *)a.vin)[row] = vin;
-        else ((float *)a.vin)[row] = vin == INFINITY ? INFINITY : f32_up(vin);
         if (i == 0) {
-            a.scal[si].band_first = b0;
             // Per-dataset radii of the banded eps; k_classify forms the bounds of each
             // pair from them and the pair's own r = q_i + q_j.
             uint32_t band_mask = 0;
