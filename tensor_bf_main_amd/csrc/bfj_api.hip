@@ -131,14 +131,17 @@ static int lds_for(int dtype, int kc, int rows, int n_eps)
     return 0;
 }
 
-// Default tile: the most resident blocks per CU (counted up to 3: latency hiding
-// across blocks matters more than strip width), then the widest strip.
+// Default tile: the most resident blocks per CU, counted up to a cap (latency hiding
+// across blocks matters more than strip width up to there), then the widest strip.
+// Measured on MI355X: FP16/BF16/FP64 peak at 3 blocks per CU with wider strips, FP8
+// (twice the MFMA rate, so more issue and LDS work per flop) at 4.
 static int heuristic_rows(int dtype, int kc, int n_eps)
 {
+    const int cap = dtype == BFJ_FP8 ? 4 : 3;
     int best = 0, best_occ = -1;
     for (int r : kRowOptions)
         if (bfj_join_has(dtype, kc, r)) {
-            const int occ = std::min(3, bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps)));
+            const int occ = std::min(cap, bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps)));
             if (occ >= best_occ) {
                 best = r;
                 best_occ = occ;
