@@ -91,11 +91,12 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
     log(f"warmup {args.warmup} x {dtype}")
     for _ in range(args.warmup):
         step(dtype, c_main)
-    join_ms, last = [], None
+    join_ms, kern_ms, last = [], [], None
     t0 = time.perf_counter()
     for _ in range(args.steps):
         last = step(dtype, c_main)
         join_ms.append(last.stats["ms_join"])
+        kern_ms.append(last.stats["ms_total"])
     t_main = (time.perf_counter() - t0) / args.steps
     pairs = float(n) * float(n)
     log(f"{dtype}: {t_main * 1e3:.1f} ms/step, join kernel {np.mean(join_ms):.1f} ms, "
@@ -112,6 +113,9 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
         "max_abs_d2_err": last.stats["max_abs_d2_err"],
         "max_band_halfwidth": last.stats["max_band_halfwidth"],
         "stage_ms": {k: last.stats[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
+        # Kernel-only time of a pass (prep + join + classify/refine/finalize, HIP events),
+        # median over the timed steps; ms_per_step adds the host side of each call.
+        "kernel_ms_median": float(np.median(kern_ms)),
         "tile_rows": last.stats["tile_rows"],
     }
     # Algorithmic flops: true d, FMA = 2, one dot product per unordered pair (the
