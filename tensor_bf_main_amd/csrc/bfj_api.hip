@@ -323,6 +323,9 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         p.sets.push_back(s);
     }
     if (p.sets.size() > 65535) return fail(BFJ_ERR_UNSUPPORTED, "more than 65535 datasets in one pass");
+    // Candidate records and kernels index rows of a pass with 32-bit integers.
+    if (p.rows_total > (int64_t)0x7fffffff)
+        return fail(BFJ_ERR_UNSUPPORTED, "%lld padded rows in one pass (at most 2^31 - 1)", (long long)p.rows_total);
     // Work items grouped by K-chunk count, longest strips first (greedy LPT through the queue).
     std::vector<int> kcs;
     for (auto &s : p.sets)
@@ -388,6 +391,7 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
                 }
             }
         }
+        if (items->size() > (size_t)0x7fffffff) return fail(BFJ_ERR_UNSUPPORTED, "too many join work items");
         g.n_items = (int32_t)(items->size() - g.item0);
         p.max_rows = std::max(p.max_rows, rows);
         if (g.n_items) p.groups.push_back(g);
