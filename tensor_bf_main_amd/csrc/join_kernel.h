@@ -122,12 +122,7 @@ struct JoinLds {
     // also loads the norms and the last wavefront the FP8 scales.
     static constexpr int LPT_MAX = UPW + 1 + (Tr::dtype == BFJ_FP8 ? 1 : 0);
     static constexpr int RING = NB * BUF;
-    static constexpr int RB = R / 16;
-    // Per-wavefront staging of a hot tile's accumulators (lane-private, so the
-    // candidate push can loop over row blocks without holding them in VGPRs).
-    static constexpr int STAGE = RB * BFJ_WAVE * (int)sizeof(typename Tr::acc_t);
-    static constexpr int STAGE_OFF = BFJ_DIRECT_B ? 0 : RING; // no ring when fragments go to VGPRs
-    __host__ __device__ static constexpr int total(int) { return STAGE_OFF + BFJ_JWAVES * STAGE; }
+    __host__ __device__ static constexpr int total(int) { return BFJ_DIRECT_B ? 16 : RING; }
 };
 
 // Per-wavefront cursor into the candidate stream: chunks of BFJ_CHUNK records
@@ -302,8 +297,8 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             // Rare: stream every pair with acc >= n_j (Z >= 0) as a candidate record;
             // k_classify applies the per-eps tests to Z = acc - n_j.  Only the row blocks
             // that hold such a pair are visited; in a symmetric sweep the tiles with
-            // cb < rb belong to another tile.  The accumulators go through a
-            // wavefront-private LDS area so that the loop need not keep them all live.
+            // cb < rb belong to another tile.  The row block's accumulators are picked
+            // with register selects (rb is uniform), so one push body serves them all.
             const bool sym = s.sym != 0;
             uint32_t hot = 0;
 #pragma unroll
@@ -311,16 +306,15 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
                 if (__ballot(mrb[rb] >= nj) != 0ull && !(sym && cb < wrb0 + rb)) hot |= 1u << rb;
             const int col = cb * 16 + (lane & 15);
             const bool col_ok = col < s.n;
-            acc_t *stage = reinterpret_cast<acc_t *>(lds + L::STAGE_OFF + wave * L::STAGE);
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-                if ((hot >> rb) & 1u) stage[rb * BFJ_WAVE + lane] = acc[rb];
             while (hot) {
                 const int rb = __builtin_ctz(hot);
                 hot &= hot - 1u;
                 const int blk = wrb0 + rb;
-                cand_push_block<Tr>(a, cc, stage[rb * BFJ_WAVE + lane], nj, col, col_ok, blk * 16, sym,
-                                    (uint32_t)(s.row_off + blk * 16), lane);
+                acc_t v = acc[0];
+#pragma unroll
+                for (int k = 1; k < RB; ++k)
+                    if (rb == k) v = acc[k];
+                cand_push_block<Tr>(a, cc, v, nj, col, col_ok, blk * 16, sym, (uint32_t)(s.row_off + blk * 16), lane);
             }
         }
     };
