@@ -153,7 +153,7 @@ __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &c
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int li = li0 + Tr::out_row(lane, r);
-        bal[r] = __ballot(v[r] >= nj && col_ok && (!sym || col > li));
+        bal[r] = __builtin_amdgcn_ballot_w64(v[r] >= nj && col_ok && (!sym || col > li));
         cnt[r] = (uint32_t)__popcll(bal[r]);
         nrec += cnt[r];
     }
@@ -293,7 +293,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 #pragma unroll
         for (int rb = 1; rb < RB; ++rb) m = fmax(m, mrb[rb]);
 
-        if (__builtin_expect(__ballot(m >= nj) != 0ull, 0)) {
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(m >= nj) != 0ull, 0)) {
             // Rare: stream every pair with acc >= n_j (Z >= 0) as a candidate record;
             // k_classify applies the per-eps tests to Z = acc - n_j.  Only the row blocks
             // that hold such a pair are visited; in a symmetric sweep the tiles with
@@ -302,8 +302,11 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             const bool sym = s.sym != 0;
             uint32_t hot = 0;
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-                if (__ballot(mrb[rb] >= nj) != 0ull && !(sym && cb < wrb0 + rb)) hot |= 1u << rb;
+            for (int rb = 0; rb < RB; ++rb) hot |= (uint32_t)(__builtin_amdgcn_ballot_w64(mrb[rb] >= nj) != 0ull) << rb;
+            if (sym) { // keep the row blocks rb <= cb - wrb0
+                const int lim = cb - wrb0;
+                hot &= lim < 0 ? 0u : (lim >= RB - 1 ? ~0u : (2u << lim) - 1u);
+            }
             const int col = cb * 16 + (lane & 15);
             const bool col_ok = col < s.n;
             while (hot) {
