@@ -127,6 +127,19 @@ int bfj_self_join_pairs(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, 
                         double eps, int32_t dtype, uint32_t *pairs, uint64_t capacity,
                         uint64_t *n_pairs, uint64_t *total, bfj_stats *stats);
 
+/* The reference's own arithmetic instead of the exact predicate, for bug-for-bug
+ * totals: search_mode 1 (distanceCalculationBruteForceCuda, kernel_join.cu:168-200),
+ * 2 (...CudaAlt, kernel_join.cu:203-239), 3 (...TensorBasic, kernel_join.cu:245-325)
+ * or 4 (...TensorHalf, kernel_join.cu:328-524), computed with the reference's
+ * COMPUTE_PREC (compute_prec: 16, 32 or 64) and ACCUM_PREC (accum_prec: 32 or 64),
+ * params.h:10-12.  Modes 3 and 4 exist for 16 / 32 only.  *total gets the ordered,
+ * self-inclusive count of the reference (gpu_join.cu:318-321).  Modes 3 and 4 sum each
+ * 16-column chunk's products in column order in FP32, an assumption about the tensor
+ * cores the reference ran on.  d <= 256. */
+int bfj_reference_mode_join(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, int64_t ld,
+                            double eps, int32_t search_mode, int32_t compute_prec, int32_t accum_prec,
+                            uint64_t *total);
+
 /* Many independent datasets, each with its own eps list, in one persistent launch.
  * device_resident != 0: sets[i].data / counts are device pointers. */
 int bfj_self_join_batch(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32_t dtype,

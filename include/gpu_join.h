@@ -50,6 +50,16 @@ int GPUJoinMainBruteForcePairs(unsigned int searchMode, unsigned int device, con
                                unsigned int dim, unsigned int computeDim, double epsilon,
                                std::vector<uint32_t> *pairs, uint64_t *totalNeighbors, bfj_stats *stats);
 
+/*
+ * Bug-for-bug variant (no counterpart as a function in the reference): the total of
+ * searchMode 1-4 computed with the reference's own arithmetic at computePrec /
+ * accumPrec (params.h COMPUTE_PREC / ACCUM_PREC), bfj_reference_mode_join.  Returns a
+ * bfj_status.
+ */
+int GPUJoinMainBruteForceReference(unsigned int searchMode, unsigned int device, const double *dataset, unsigned int n,
+                                   unsigned int dim, unsigned int computeDim, double epsilon, int computePrec,
+                                   int accumPrec, uint64_t *totalNeighbors);
+
 /* searchMode -> bfj_dtype, or -1 for modes that are errors (5, 6, unknown). */
 int searchModeDtype(unsigned int searchMode);
 

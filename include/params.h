@@ -44,6 +44,13 @@
 #define C 16
 #endif
 
+/* 0: searchModes 1-4 return the exact FP64 answer.  1: they count with the
+ * reference's own COMPUTE_PREC / ACCUM_PREC arithmetic instead (bug-for-bug totals,
+ * bfj_reference_mode_join).  Not in the reference, which only has the latter. */
+#ifndef BFJ_REFERENCE_ARITHMETIC
+#define BFJ_REFERENCE_ARITHMETIC 0
+#endif
+
 #ifndef ADDITIONAL_POINTS
 #define ADDITIONAL_POINTS 15 /* params.h:20: zero rows appended by the driver */
 #endif
@@ -55,7 +62,8 @@
 #define SEARCHMODE_ARG 3
 #define DEVICE_ARG 4
 
-/* params.h:166-171: search modes.  Every mode returns the exact FP64 answer. */
+/* params.h:166-171: search modes.  Every mode returns the exact FP64 answer unless
+ * BFJ_REFERENCE_ARITHMETIC is set. */
 #define SM_GPU 1       /* reference: CUDA-core direct form     -> here FP64 MFMA + FP64 band refine */
 #define SM_CUDA_ALT 2  /* reference: CUDA-core expanded form   -> here FP64 MFMA + FP64 band refine */
 #define SM_TENSOR 3    /* reference: WMMA identity trick, half -> here FP16 MFMA + FP64 band refine */

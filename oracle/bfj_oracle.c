@@ -196,3 +196,160 @@ int bfo_num_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------------
+ * The reference's own arithmetic for its search modes 1-4, as a checker for the
+ * library's bfj_reference_mode_join.  Values of every type are held in doubles
+ * (binary16 and binary32 values are exact there) and every operation is rounded
+ * once to the type it runs in:
+ *   mode 1  kernel_join.cu:168-200   t = C(q_k - c_k); acc = A(acc + A(C(t * t)))
+ *   mode 2  kernel_join.cu:203-239   acc = A(acc + A(A(A(C(C(-2 q_k) c_k)) + A(q_k)A(q_k)) + A(C(c_k c_k))))
+ *   mode 3  kernel_join.cu:245-325   per 16-column chunk: D_k = FP16(q_k - c_k),
+ *                                    acc = acc + (sum of D_k^2 in column order, FP32)
+ *   mode 4  kernel_join.cu:328-524   per chunk: t = cn_c + (sum of FP16(-2 q_k) c_k, FP32),
+ *                                    acc = (acc + t) + cn_q; cn = FP32 chunk sums of squares
+ *                                    (kernel_join.cu:78-86)
+ * When the compute and accumulation types are the same, nvcc's default contraction
+ * (-fmad=true) fuses a product into the sum that consumes it: mode 1 becomes
+ * acc = fma(t, t, acc) (the FP64 oracle above for 64/64) and mode 2
+ * acc = acc + fma(c, c, fma(C(-2 q), c, q2)).  Modes 3 and 4 ran on tensor cores whose
+ * summation order is undocumented; the column-order FP32 sum is an assumption, so those
+ * two restate the library's definition rather than the reference's hardware.
+ * The hit test sqrt(acc) <= A(eps) (modes 2 and 4 on |acc|) is acc <= T_A. */
+
+/* Round to binary16, nearest even, from a double in one step (x86 has no direct
+ * conversion; going through float first could round twice). */
+static double bfo_h16(double x)
+{
+    if (isnan(x)) return x;
+    const double ax = fabs(x);
+    if (ax >= 65520.0) return copysign(INFINITY, x); /* 65504 + half an ulp rounds up to inf */
+    double ulp;
+    if (ax < 0x1p-14) ulp = 0x1p-24; /* subnormal binade */
+    else {
+        int e;
+        (void)frexp(ax, &e); /* ax in [2^(e-1), 2^e) */
+        ulp = ldexp(1.0, e - 1 - 10);
+    }
+    return nearbyint(x / ulp) * ulp; /* exact scaling; nearbyint rounds half to even */
+}
+
+/* Exported for the tests that pin the rounding against numpy's float64 -> float16. */
+double bfo_round_half(double x) { return bfo_h16(x); }
+
+static double bfo_to_c(double x, int cp) { return cp == 16 ? bfo_h16(x) : cp == 32 ? (double)(float)x : x; }
+static double bfo_to_a(double x, int ap) { return ap == 32 ? (double)(float)x : x; }
+/* a - b and a * b in the compute type (operands already of that type). */
+static double bfo_csub(double a, double b, int cp)
+{
+    if (cp == 16) return bfo_h16(a - b); /* exact in double, then one rounding */
+    if (cp == 32) return (double)((float)a - (float)b);
+    return a - b;
+}
+static double bfo_cmul(double a, double b, int cp)
+{
+    if (cp == 16) return bfo_h16(a * b);
+    if (cp == 32) return (double)((float)a * (float)b);
+    return a * b;
+}
+static double bfo_aadd(double a, double b, int ap) { return ap == 32 ? (double)((float)a + (float)b) : a + b; }
+static double bfo_amul(double a, double b, int ap) { return ap == 32 ? (double)((float)a * (float)b) : a * b; }
+static double bfo_afma(double a, double b, double c, int ap)
+{
+    return ap == 32 ? (double)fmaf((float)a, (float)b, (float)c) : fma(a, b, c);
+}
+
+static double bfo_threshold_f(double eps)
+{
+    const float e = (float)eps;
+    if (isnan(e)) return NAN;
+    if (e < 0.0f) return -1.0;
+    if (isinf(e)) return INFINITY;
+    float t = e * e;
+    if (isinf(t)) t = FLT_MAX;
+    while (t > 0.0f && sqrtf(t) > e) t = nextafterf(t, -INFINITY);
+    for (;;) {
+        float u = nextafterf(t, INFINITY);
+        if (isinf(u) || sqrtf(u) > e) break;
+        t = u;
+    }
+    return (double)t;
+}
+
+int bfo_reference_total(const double *data, int64_t n, int32_t d, int64_t ld, double eps, int32_t mode, int32_t cp,
+                        int32_t ap, uint64_t *total)
+{
+    if (!total || n < 0 || d < 1 || ld < d || (n > 0 && !data)) return 1;
+    if (mode < 1 || mode > 4) return 2;
+    if ((cp != 16 && cp != 32 && cp != 64) || (ap != 32 && ap != 64)) return 2;
+    if (mode >= 3 && (cp != 16 || ap != 32)) return 2;
+    const int dpad = (d + 15) / 16 * 16, nch = dpad / 16;
+    const double thr = ap == 32 ? bfo_threshold_f(eps) : bfo_threshold(eps);
+    double *X = (double *)calloc((size_t)(n > 0 ? n : 1) * dpad, sizeof(double));
+    double *cn = (double *)calloc((size_t)(n > 0 ? n : 1) * nch, sizeof(double));
+    if (!X || !cn) {
+        free(X);
+        free(cn);
+        return 3;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < d; ++k) X[i * dpad + k] = bfo_to_c(data[i * ld + k], cp); /* convertDataset */
+    for (int64_t i = 0; i < n; ++i)
+        for (int ch = 0; ch < nch; ++ch) {
+            float s = 0.0f;
+            for (int r = 0; r < 16; ++r) {
+                const float v = (float)X[i * dpad + ch * 16 + r];
+                s = fmaf(v, v, s); /* v*v is exact, so fused or not gives the same sum */
+            }
+            cn[i * nch + ch] = (double)s;
+        }
+    uint64_t hits = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : hits)
+    for (int64_t i = 0; i < n; ++i) {
+        const double *q = X + i * dpad;
+        for (int64_t j = 0; j < n; ++j) {
+            const double *c = X + j * dpad;
+            double acc = 0.0;
+            if (mode == 1) {
+                for (int k = 0; k < d; ++k) {
+                    const double t = bfo_csub(q[k], c[k], cp);
+                    acc = cp == ap ? bfo_afma(t, t, acc, ap) : bfo_aadd(acc, bfo_to_a(bfo_cmul(t, t, cp), ap), ap);
+                }
+            } else if (mode == 2) {
+                for (int k = 0; k < d; ++k) {
+                    const double qm2 = bfo_cmul(-2.0, q[k], cp);
+                    const double q2 = bfo_amul(bfo_to_a(q[k], ap), bfo_to_a(q[k], ap), ap);
+                    double t;
+                    if (cp == ap) t = bfo_afma(c[k], c[k], bfo_afma(qm2, c[k], q2, ap), ap);
+                    else
+                        t = bfo_aadd(bfo_aadd(bfo_to_a(bfo_cmul(qm2, c[k], cp), ap), q2, ap),
+                                     bfo_to_a(bfo_cmul(c[k], c[k], cp), ap), ap);
+                    acc = bfo_aadd(acc, t, ap);
+                }
+            } else {
+                for (int ch = 0; ch < nch; ++ch) {
+                    float s = 0.0f;
+                    for (int r = 0; r < 16; ++r) {
+                        const int k = ch * 16 + r;
+                        float p;
+                        if (mode == 3) {
+                            const float dd = (float)bfo_h16(q[k] - c[k]);
+                            p = dd * dd;
+                        } else {
+                            p = (float)bfo_h16(-2.0 * q[k]) * (float)c[k];
+                        }
+                        s = r == 0 ? p : s + p;
+                    }
+                    if (mode == 3) acc = (double)((float)acc + s);
+                    else acc = (double)(((float)acc + ((float)cn[j * nch + ch] + s)) + (float)cn[i * nch + ch]);
+                }
+            }
+            const double v = (mode == 2 || mode == 4) ? fabs(acc) : acc;
+            if (v <= thr) ++hits;
+        }
+    }
+    free(X);
+    free(cn);
+    *total = hits;
+    return 0;
+}

@@ -68,6 +68,11 @@ def _load():
                              ctypes.c_int64, ctypes.c_int64, i64p, ctypes.POINTER(ctypes.c_uint64),
                              ctypes.c_int32]
     lib.bfo_num_threads.restype = ctypes.c_int
+    lib.bfo_round_half.restype = ctypes.c_double
+    lib.bfo_round_half.argtypes = [ctypes.c_double]
+    lib.bfo_reference_total.restype = ctypes.c_int
+    lib.bfo_reference_total.argtypes = [f64p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, ctypes.c_double,
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]
     _lib = lib
     return lib
 
@@ -135,6 +140,26 @@ def join(data, eps, row_begin: int = 0, row_end: int | None = None, nthreads: in
     if rc:
         raise ValueError(f"bfo_join failed ({rc})")
     return counts, totals
+
+
+def round_half(x: float) -> float:
+    """binary16 rounding (nearest even) of a double, as the reference-arithmetic
+    restatement uses it."""
+    return float(_load().bfo_round_half(float(x)))
+
+
+def reference_total(data, eps: float, mode: int, compute_prec: int = 16, accum_prec: int = 32) -> int:
+    """The reference's own total for search mode 1-4 with its COMPUTE_PREC / ACCUM_PREC
+    arithmetic (kernel_join.cu:168-524 restated in bfj_oracle.c, bfo_reference_total):
+    ordered pairs, self included."""
+    data = _as_data(data)
+    n, d = data.shape
+    tot = ctypes.c_uint64(0)
+    rc = _load().bfo_reference_total(_ptr(data, ctypes.c_double), n, d, d, float(eps), int(mode), int(compute_prec),
+                                     int(accum_prec), ctypes.byref(tot))
+    if rc:
+        raise ValueError(f"bfo_reference_total failed ({rc})")
+    return int(tot.value)
 
 
 _NUMBER = re.compile(r"\s*([+-]?(?:(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|inf(?:inity)?|nan))", re.IGNORECASE)

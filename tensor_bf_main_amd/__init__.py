@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH):
         "bfj_self_join_batch": (ctypes.c_int, [vp, ctypes.POINTER(Dataset), i32, i32, i32, sp]),
         "bfj_self_join_pairs": (ctypes.c_int, [vp, vp, i64, i32, i64, ctypes.c_double, i32, vp, ctypes.c_uint64,
                                                u64p, u64p, sp]),
+        "bfj_reference_mode_join": (ctypes.c_int, [vp, vp, i64, i32, i64, ctypes.c_double, i32, i32, i32, u64p]),
         "bfj_plan_create": (ctypes.c_int, [vp, ctypes.POINTER(Dataset), i32, i32, ctypes.POINTER(vp)]),
         "bfj_plan_run": (ctypes.c_int, [vp, sp]),
         "bfj_plan_destroy": (ctypes.c_int, [vp]),
@@ -285,6 +286,21 @@ class Context:
             _check(rc)
             return buf[: found.value], int(tot.value), st.as_dict()
         raise JoinError(BFJ_ERR_CAPACITY, "pair buffer still too small after growing it")
+
+    def reference_mode_join(self, data, eps: float, search_mode: int, compute_prec: int = 16,
+                            accum_prec: int = 32) -> int:
+        """The reference's own total for ``search_mode`` 1-4 (ordered pairs, self
+        included), computed with its COMPUTE_PREC / ACCUM_PREC arithmetic (params.h:10-12)
+        rather than the exact FP64 predicate; see ``bfj_reference_mode_join``."""
+        x = np.ascontiguousarray(data, dtype=np.float64)
+        if x.ndim != 2:
+            raise ValueError("data must be a 2-D float64 array")
+        n, d = x.shape
+        tot = ctypes.c_uint64(0)
+        _check(self.lib.bfj_reference_mode_join(self._h, x.ctypes.data if n else None, n, d, d, float(eps),
+                                                int(search_mode), int(compute_prec), int(accum_prec),
+                                                ctypes.byref(tot)))
+        return int(tot.value)
 
     def to_device(self, host, dtype=np.float64) -> DeviceArray:
         a = np.ascontiguousarray(host, dtype=dtype)

@@ -56,6 +56,8 @@ static int fail(int code, const char *fmt, ...)
 
 // ---------------------------------------------------------------------------
 // dispatch to the per-dtype translation units
+int bfj_compat_run(const double *x, int64_t n, int d, int64_t ld, double eps, int mode, int cprec, int aprec, void *X,
+                   float *cn, unsigned long long *cnt, hipStream_t st);
 int bfj_join_launch_f16(const JoinLaunch &, const JoinArgs &, hipStream_t);
 int bfj_join_launch_bf16(const JoinLaunch &, const JoinArgs &, hipStream_t);
 int bfj_join_launch_f8(const JoinLaunch &, const JoinArgs &, hipStream_t);
@@ -186,7 +188,7 @@ struct bfj_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
     DevBuf pack, nq, gq, he, row_set, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
-        outbuf, cand, chunk_fill, pairs;
+        outbuf, cand, chunk_fill, pairs, ref_x, ref_aux;
     uint32_t band_capacity = 0;
     uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
     int emit_pairs = 0;
@@ -712,7 +714,7 @@ int bfj_destroy(bfj_ctx *c)
     (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
                       &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
-                      &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs};
+                      &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -851,6 +853,42 @@ static int batch_hostsets(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets,
         if (s.n > 0) ob += (size_t)s.n * s.n_eps * 8;
     }
     (void)ctx;
+    return BFJ_OK;
+}
+
+int bfj_reference_mode_join(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, int64_t ld, double eps,
+                            int32_t search_mode, int32_t compute_prec, int32_t accum_prec, uint64_t *total)
+{
+    RC(common_checks(ctx, data, n, &eps, 1));
+    if (!total) return fail(BFJ_ERR_INVALID, "total is NULL");
+    if (d < 1 || ld < d) return fail(BFJ_ERR_INVALID, "need 1 <= d <= ld");
+    if (d > BFJ_MAXDIM) return fail(BFJ_ERR_UNSUPPORTED, "d = %d > %d", d, BFJ_MAXDIM);
+    if (n > (int64_t)0x7fffffff) return fail(BFJ_ERR_UNSUPPORTED, "n > 2^31-1");
+    const bool prec_ok = (compute_prec == 16 || compute_prec == 32 || compute_prec == 64) &&
+                         (accum_prec == 32 || accum_prec == 64);
+    if (search_mode < 1 || search_mode > 4) return fail(BFJ_ERR_INVALID, "search_mode %d is not 1..4", search_mode);
+    if (!prec_ok || (search_mode >= 3 && (compute_prec != 16 || accum_prec != 32)))
+        return fail(BFJ_ERR_UNSUPPORTED, "search_mode %d has no reference arithmetic for compute %d / accum %d",
+                    search_mode, compute_prec, accum_prec);
+    *total = 0;
+    if (n == 0) return BFJ_OK;
+    const int dpad = (d + 15) / 16 * 16;
+    RC(ensure(ctx->xcopy, (size_t)n * ld * 8));
+    HC(hipMemcpyAsync(ctx->xcopy.p, data, (size_t)n * ld * 8, hipMemcpyHostToDevice, ctx->stream));
+    RC(ensure(ctx->ref_x, (size_t)n * dpad * (compute_prec / 8)));
+    // Hit counter first, then the FP32 chunk norms of mode 4.
+    RC(ensure(ctx->ref_aux, 256 + (size_t)n * (dpad / 16) * sizeof(float)));
+    bfj_launch_zero(ctx->ref_aux.p, 8, ctx->stream);
+    unsigned long long *cnt = (unsigned long long *)ctx->ref_aux.p;
+    float *cn = (float *)((char *)ctx->ref_aux.p + 256);
+    const int rc = bfj_compat_run((const double *)ctx->xcopy.p, n, d, ld, eps, search_mode, compute_prec, accum_prec,
+                                  ctx->ref_x.p, cn, cnt, ctx->stream);
+    if (rc == 1) return fail(BFJ_ERR_UNSUPPORTED, "no reference-arithmetic kernel for this combination");
+    if (rc != 0) return fail(BFJ_ERR_DEVICE, "reference-arithmetic launch failed");
+    unsigned long long h = 0;
+    HC(hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    *total = (uint64_t)h;
     return BFJ_OK;
 }
 

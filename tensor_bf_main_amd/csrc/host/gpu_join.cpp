@@ -107,6 +107,23 @@ int GPUJoinMainBruteForcePairs(unsigned int searchMode, unsigned int device, con
     return BFJ_OK;
 }
 
+int GPUJoinMainBruteForceReference(unsigned int searchMode, unsigned int device, const double *dataset, unsigned int n,
+                                   unsigned int dim, unsigned int computeDim, double epsilon, int computePrec,
+                                   int accumPrec, uint64_t *totalNeighbors)
+{
+    bfj_ctx *ctx = context_for(device);
+    if (!ctx) return BFJ_ERR_DEVICE;
+    uint64_t total = 0;
+    const int rc = bfj_reference_mode_join(ctx, dataset, (int64_t)n, (int32_t)dim, (int64_t)computeDim, epsilon,
+                                           (int32_t)searchMode, computePrec, accumPrec, &total);
+    if (rc != BFJ_OK) {
+        std::fprintf(stderr, "[GPU] ~ Error: %s\n", bfj_last_error());
+        return rc;
+    }
+    if (totalNeighbors) *totalNeighbors = total;
+    return BFJ_OK;
+}
+
 void GPUJoinMainBruteForce(unsigned int searchMode, unsigned int device, INPUT_DATA_TYPE *dataset,
                            unsigned int *nbQueryPoints, ACCUM_TYPE *epsilon, uint64_t *totalNeighbors)
 {
@@ -115,6 +132,11 @@ void GPUJoinMainBruteForce(unsigned int searchMode, unsigned int device, INPUT_D
         std::exit(1);
     }
     *totalNeighbors = 0;
+    if (BFJ_REFERENCE_ARITHMETIC && searchMode >= SM_GPU && searchMode <= SM_TENSOR_MQ) {
+        (void)GPUJoinMainBruteForceReference(searchMode, device, dataset, *nbQueryPoints, INPUT_DATA_DIM, COMPUTE_DIM,
+                                             (double)*epsilon, COMPUTE_PREC, ACCUM_PREC, totalNeighbors);
+        return;
+    }
     (void)GPUJoinMainBruteForceEx(searchMode, device, dataset, *nbQueryPoints, INPUT_DATA_DIM, COMPUTE_DIM,
                                   (double)*epsilon, totalNeighbors, nullptr, nullptr);
 }

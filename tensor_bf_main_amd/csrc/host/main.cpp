@@ -2,6 +2,7 @@
 //
 //   bfjoin_main <file> <eps> <searchMode> [device] [--dim D] [--compute-dim C]
 //               [--binary] [--counts out.txt] [--pairs out.txt] [--log file] [--parse-only]
+//               [--reference-arithmetic [--compute-prec P] [--accum-prec P]]
 //
 // Same positional arguments, console lines and CSV result log as the reference;
 // the options give the row length at run time (the reference fixes it at compile
@@ -40,7 +41,8 @@ int main(int argc, char *argv[])
     if (argc < 4) {
         std::cerr << "usage: " << argv[0]
                   << " <file> <eps> <searchMode> [device] [--dim D] [--compute-dim C] [--binary]"
-                     " [--counts out.txt] [--pairs out.txt] [--log file]\n";
+                     " [--counts out.txt] [--pairs out.txt] [--log file]"
+                     " [--reference-arithmetic [--compute-prec P] [--accum-prec P]]\n";
         return 1;
     }
     const char *filename = argv[FILENAME_ARG];
@@ -50,6 +52,8 @@ int main(int argc, char *argv[])
     int dim = INPUT_DATA_DIM, computeDim = -1;
     bool binary = false, parse_only = false;
     std::string counts_path, pairs_path, log_path = "tensor_brute-force.txt";
+    bool ref_arith = BFJ_REFERENCE_ARITHMETIC != 0;
+    int computePrec = COMPUTE_PREC, accumPrec = ACCUM_PREC;
     int pos = 0;
     for (int a = 4; a < argc; ++a) {
         if (!std::strcmp(argv[a], "--dim") && a + 1 < argc) dim = std::atoi(argv[++a]);
@@ -59,6 +63,9 @@ int main(int argc, char *argv[])
         else if (!std::strcmp(argv[a], "--counts") && a + 1 < argc) counts_path = argv[++a];
         else if (!std::strcmp(argv[a], "--pairs") && a + 1 < argc) pairs_path = argv[++a];
         else if (!std::strcmp(argv[a], "--log") && a + 1 < argc) log_path = argv[++a];
+        else if (!std::strcmp(argv[a], "--reference-arithmetic")) ref_arith = true;
+        else if (!std::strcmp(argv[a], "--compute-prec") && a + 1 < argc) computePrec = std::atoi(argv[++a]);
+        else if (!std::strcmp(argv[a], "--accum-prec") && a + 1 < argc) accumPrec = std::atoi(argv[++a]);
         else if (pos++ == 0) device = (unsigned)std::atoi(argv[a]); // DEVICE_ARG
         else {
             std::cerr << "[Main] ~ Error: unknown argument " << argv[a] << "\n";
@@ -128,9 +135,18 @@ int main(int argc, char *argv[])
     std::memset(&stats, 0, sizeof stats);
     uint64_t totalResult = 0;
     const double tJoin0 = now_s();
-    const int rc = GPUJoinMainBruteForceEx(searchMode, device, database.data(), nbQueryPoints, (unsigned)dim,
-                                           (unsigned)computeDim, epsilon, &totalResult,
-                                           counts.empty() ? nullptr : counts.data(), &stats);
+    const bool ref_mode = ref_arith && searchMode >= SM_GPU && searchMode <= SM_TENSOR_MQ;
+    if (ref_mode && (!counts_path.empty() || !pairs_path.empty())) {
+        std::cerr << "[Main] ~ Error: the reference's arithmetic gives a total only (no --counts / --pairs)\n";
+        return 1;
+    }
+    // The reference passes eps as ACCUM_TYPE (gpu_join.cu:270); the library rounds it.
+    const int rc = ref_mode ? GPUJoinMainBruteForceReference(searchMode, device, database.data(), nbQueryPoints,
+                                                             (unsigned)dim, (unsigned)computeDim, epsilon, computePrec,
+                                                             accumPrec, &totalResult)
+                            : GPUJoinMainBruteForceEx(searchMode, device, database.data(), nbQueryPoints,
+                                                      (unsigned)dim, (unsigned)computeDim, epsilon, &totalResult,
+                                                      counts.empty() ? nullptr : counts.data(), &stats);
     const double timeJoin = now_s() - tJoin0;
     if (rc != BFJ_OK) return 1;
     std::cout << "[Main | Result] ~ Time to join: " << timeJoin << '\n';
@@ -164,8 +180,8 @@ int main(int argc, char *argv[])
                "warpPerBlock, computePrec, accumPrec, dtype, pairsPerSec, bandPairs, maxAbsD2Err, tileRows, "
                "kernelMs\n";
     out << filename << ", " << epsilon << ", " << searchMode << ", " << timeJoin << ", " << totalResult << ", " << dim
-        << ", " << computeDim << ", " << BLOCKSIZE << ", " << WARP_PER_BLOCK << ", " << COMPUTE_PREC << ", "
-        << ACCUM_PREC << ", " << dtype_name(stats.dtype) << ", " << (timeJoin > 0 ? pairs / timeJoin : 0.0) << ", "
+        << ", " << computeDim << ", " << BLOCKSIZE << ", " << WARP_PER_BLOCK << ", " << computePrec << ", "
+        << accumPrec << ", " << (ref_mode ? "reference" : dtype_name(stats.dtype)) << ", " << (timeJoin > 0 ? pairs / timeJoin : 0.0) << ", "
         << stats.band_pairs << ", " << stats.max_abs_d2_err << ", " << stats.tile_rows << ", " << stats.ms_total
         << std::endl;
     return 0;
