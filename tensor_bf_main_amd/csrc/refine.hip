@@ -206,86 +206,80 @@ __device__ inline double fma_chunk(const double (&u)[RK], const double (&v)[RK],
     return acc;
 }
 
-__global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
+// Refine one band record per lane (lanes with live == false take part in the votes
+// and decide nothing): the sequential FMA chain of the pair's two FP64 rows, the
+// bucket of the first eps (ascending) whose threshold holds it, counted for its row
+// and (symmetric record) for the other row, and the approximation error for the
+// statistics.  All lanes take part.
+__device__ inline void refine_one(const PassArgs &a, const BandEntry &be, bool live, int lane, int &err_set,
+                                  double &err_run)
 {
-    const int lane = threadIdx.x & 63;
-    const uint32_t cnt = *a.band_count;
-    const uint32_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
-    const uint32_t stride = gridDim.x * BFJ_BLOCK;
-    // Largest approximation error seen by this lane, per dataset: kept in a register
-    // while the dataset does not change, flushed once (one atomic on a counter every
-    // wavefront shares would serialise the whole kernel).
-    int err_set = -1;
-    double err_run = -1.0;
-    // base is uniform over the wavefront, so every lane takes part in the votes below.
-    for (uint32_t base = blockIdx.x * BFJ_BLOCK + (threadIdx.x & ~63u); base < m; base += stride) {
-        const uint32_t t = base + lane;
-        const bool live = t < m;
-        BandEntry be;
-        be.i = be.j = 0;
-        be.z = 0.f;
-        be.meta = 0;
-        if (live) be = a.band[t];
-        const int si = (int)(be.meta >> 16);
-        const SetDesc &s = a.sets[si];
-        const int d = live ? s.d : 0;
-        const bfj_gptr p = (bfj_gptr)(s.x + (int64_t)be.i * s.ld);
-        const bfj_gptr q = (bfj_gptr)(s.x + (int64_t)be.j * s.ld);
-        // Each lane runs the sequential FMA chain of its own pair.  Whole chunks go
-        // through two register buffers, the next chunk's loads in flight while the
-        // current one is consumed; the last d % RK values are loaded one by one.
-        double acc = 0.0;
-        const int nfull = d / RK;
-        double u0[RK], v0[RK], u1[RK], v1[RK];
-        const bool al16 = s.al16 != 0;
-        if (nfull > 0) {
-            load_chunk(p, al16, u0);
-            load_chunk(q, al16, v0);
-        }
-        for (int c = 0; c < nfull; c += 2) {
-            if (c + 1 < nfull) {
-                load_chunk(p + (c + 1) * RK, al16, u1);
-                load_chunk(q + (c + 1) * RK, al16, v1);
-            }
-            acc = fma_chunk(u0, v0, acc);
-            if (c + 1 >= nfull) break;
-            if (c + 2 < nfull) {
-                load_chunk(p + (c + 2) * RK, al16, u0);
-                load_chunk(q + (c + 2) * RK, al16, v0);
-            }
-            acc = fma_chunk(u1, v1, acc);
-        }
-        for (int k = nfull * RK; k < d; ++k) {
-            const double tk = p[k] - q[k];
-            acc = fma(tk, tk, acc);
-        }
-        double err = -1.0;
-        // The pair counts once, under the first eps (ascending) whose threshold holds
-        // its FP64 distance, for its row and (symmetric record) for the other row.
-        int k = -1;
-        if (live)
-            for (int e = 0; e < s.n_eps; ++e)
-                if (acc <= s.T[e]) {
-                    k = e;
-                    break;
-                }
-        add_bucket(a, (uint32_t)s.row_off + be.i, s, be.i, k, lane);
-        add_bucket(a, (uint32_t)s.row_off + be.j, s, be.j, (be.meta & 0x8000u) ? k : -1, lane);
-        if (a.emit_pairs) append_pairs(a, k == 0 && be.i != be.j, be.i, be.j, lane);
-        if (live) {
-            const double sigma = a.scal[si].sigma;
-            const double approx = sigma * sigma * (a.he[s.row_off + be.i].x - 2.0 * (double)be.z);
-            err = fabs(approx - acc);
-            if (si != err_set) {
-                if (err_set >= 0 && err_run >= 0.0)
-                    atomicMax(&a.scal[err_set].err_bits, (unsigned long long)__double_as_longlong(err_run));
-                err_set = si;
-                err_run = -1.0;
-            }
-            err_run = fmax(err_run, err);
-        }
+    const int si = (int)(be.meta >> 16);
+    const SetDesc &s = a.sets[si];
+    const int d = live ? s.d : 0;
+    const bfj_gptr p = (bfj_gptr)(s.x + (int64_t)be.i * s.ld);
+    const bfj_gptr q = (bfj_gptr)(s.x + (int64_t)be.j * s.ld);
+    // Each lane runs the sequential FMA chain of its own pair.  Whole chunks go
+    // through two register buffers, the next chunk's loads in flight while the
+    // current one is consumed; the last d % RK values are loaded one by one.
+    double acc = 0.0;
+    const int nfull = d / RK;
+    double u0[RK], v0[RK], u1[RK], v1[RK];
+    const bool al16 = s.al16 != 0;
+    if (nfull > 0) {
+        load_chunk(p, al16, u0);
+        load_chunk(q, al16, v0);
     }
-    // Flush: one atomic per wavefront for the common case of one dataset.
+    for (int c = 0; c < nfull; c += 2) {
+        if (c + 1 < nfull) {
+            load_chunk(p + (c + 1) * RK, al16, u1);
+            load_chunk(q + (c + 1) * RK, al16, v1);
+        }
+        acc = fma_chunk(u0, v0, acc);
+        if (c + 1 >= nfull) break;
+        if (c + 2 < nfull) {
+            load_chunk(p + (c + 2) * RK, al16, u0);
+            load_chunk(q + (c + 2) * RK, al16, v0);
+        }
+        acc = fma_chunk(u1, v1, acc);
+    }
+    for (int k = nfull * RK; k < d; ++k) {
+        const double tk = p[k] - q[k];
+        acc = fma(tk, tk, acc);
+    }
+    // The pair counts once, under the first eps (ascending) whose threshold holds
+    // its FP64 distance, for its row and (symmetric record) for the other row.
+    int k = -1;
+    if (live)
+        for (int e = 0; e < s.n_eps; ++e)
+            if (acc <= s.T[e]) {
+                k = e;
+                break;
+            }
+    add_bucket(a, (uint32_t)s.row_off + be.i, s, be.i, k, lane);
+    add_bucket(a, (uint32_t)s.row_off + be.j, s, be.j, (be.meta & 0x8000u) ? k : -1, lane);
+    if (a.emit_pairs) append_pairs(a, k == 0 && be.i != be.j, be.i, be.j, lane);
+    if (live) {
+        const double sigma = a.scal[si].sigma;
+        const double approx = sigma * sigma * (a.he[s.row_off + be.i].x - 2.0 * (double)be.z);
+        const double err = fabs(approx - acc);
+        // Largest approximation error per dataset, kept in a register while the
+        // dataset does not change and flushed by refine_flush (one atomic on a
+        // counter every wavefront shares would serialise the whole kernel).
+        if (si != err_set) {
+            if (err_set >= 0 && err_run >= 0.0)
+                atomicMax(&a.scal[err_set].err_bits, (unsigned long long)__double_as_longlong(err_run));
+            err_set = si;
+            err_run = -1.0;
+        }
+        err_run = fmax(err_run, err);
+    }
+}
+
+// Flush of the running error maxima: one atomic per wavefront for the common case
+// of one dataset.  All lanes take part.
+__device__ inline void refine_flush(const PassArgs &a, int err_set, double err_run, int lane)
+{
     const int set0 = __shfl(err_set, 0);
     if (__ballot(err_set != set0) == 0ull) {
         double mx = err_run;
@@ -295,6 +289,28 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
     } else if (err_set >= 0 && err_run >= 0.0) {
         atomicMax(&a.scal[err_set].err_bits, (unsigned long long)__double_as_longlong(err_run));
     }
+}
+
+__global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t cnt = *a.band_count;
+    const uint32_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
+    const uint32_t stride = gridDim.x * BFJ_BLOCK;
+    int err_set = -1;
+    double err_run = -1.0;
+    // base is uniform over the wavefront, so every lane takes part in the votes.
+    for (uint32_t base = blockIdx.x * BFJ_BLOCK + (threadIdx.x & ~63u); base < m; base += stride) {
+        const uint32_t t = base + lane;
+        const bool live = t < m;
+        BandEntry be;
+        be.i = be.j = 0;
+        be.z = 0.f;
+        be.meta = 0;
+        if (live) be = a.band[t];
+        refine_one(a, be, live, lane, err_set, err_run);
+    }
+    refine_flush(a, err_set, err_run, lane);
 }
 
 // Pair-list mode when eps covers every pair: all (i, j), i < j, row i's pairs at
