@@ -164,6 +164,10 @@ int bfj_buffer_download(bfj_ctx *ctx, void *h_dst, const void *d_src, uint64_t b
  * joins use it.  bfj_set_tile_rows forces a shape (0 = heuristic). */
 int bfj_autotune(bfj_ctx *ctx, int32_t dtype, int32_t d, int64_t n, int32_t *tile_rows_out);
 int bfj_set_tile_rows(bfj_ctx *ctx, int32_t dtype, int32_t tile_rows);
+/* Locality order (no counterpart in the reference; default on): a pass whose rows
+ * are all queries packs them grouped by nearest of 128 pivot rows, so the pairs near
+ * eps fall into few tiles.  Results do not depend on it; enable = 0 keeps input order. */
+int bfj_set_reorder(bfj_ctx *ctx, int32_t enable);
 int bfj_tile_rows_options(int32_t dtype, int32_t d, int32_t *options, int32_t max_options, int32_t *n_options);
 
 #ifdef __cplusplus

@@ -104,6 +104,7 @@ def load_library(path: str = LIB_PATH):
         "bfj_plan_destroy": (ctypes.c_int, [vp]),
         "bfj_autotune": (ctypes.c_int, [vp, i32, i32, i64, ctypes.POINTER(i32)]),
         "bfj_set_tile_rows": (ctypes.c_int, [vp, i32, i32]),
+        "bfj_set_reorder": (ctypes.c_int, [vp, i32]),
         "bfj_tile_rows_options": (ctypes.c_int, [i32, i32, ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]),
         "bfj_buffer_alloc": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(vp)]),
         "bfj_buffer_free": (ctypes.c_int, [vp, vp]),
@@ -377,6 +378,11 @@ class Context:
 
     def set_tile_rows(self, dtype, rows: int):
         _check(self.lib.bfj_set_tile_rows(self._h, _dtype_id(dtype), rows))
+
+    def set_reorder(self, enable: bool):
+        """Turn the locality order of whole-dataset passes on (default) or off; results
+        are the same either way."""
+        _check(self.lib.bfj_set_reorder(self._h, 1 if enable else 0))
 
 
 class Plan:

@@ -188,7 +188,8 @@ struct bfj_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
     DevBuf pack, nq, gq, he, row_set, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
-        outbuf, cand, chunk_fill, pairs, ref_x, ref_aux;
+        outbuf, cand, chunk_fill, pairs, ref_x, ref_aux, perm, label, ohist, oseg, pivots, pivot_norm;
+    int reorder = 1; // pack whole-dataset passes in a locality order (order.hip)
     uint32_t band_capacity = 0;
     uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
     int emit_pairs = 0;
@@ -295,6 +296,8 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         s.set_id = (int32_t)si;
         s.sym = (h.row_begin == 0 && h.row_end == h.n) ? 1 : 0;
         s.al16 = ((uintptr_t)h.x % 16 == 0 && h.ld % 2 == 0) ? 1 : 0;
+        // A locality order only pays once a dataset spans many tiles.
+        s.reorder = (ctx->reorder && s.sym && h.n >= 512) ? 1 : 0;
         s.out = h.out;
         // Thresholds in ascending order: a pair within eps k is within every later one,
         // so the kernels count each pair once, under the first eps that holds it.
@@ -446,6 +449,15 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     const size_t nb = p.blk_set.size(), ns = p.sets.size();
     const size_t tab_bytes = nb * 4 + nb * 8 + ns * 4 * 2 + 64;
     RC(ensure(ctx->tabs, tab_bytes));
+    int max_nblk = 1;
+    for (int32_t v : p.set_nblk) max_nblk = std::max(max_nblk, (int)v);
+    const size_t max_seg = (size_t)(max_nblk + BFJ_ORD_SEG - 1) / BFJ_ORD_SEG;
+    RC(ensure(ctx->perm, nrows * 4));
+    RC(ensure(ctx->label, nrows));
+    RC(ensure(ctx->ohist, std::max<size_t>(nb, 1) * BFJ_PIVOTS * 4));
+    RC(ensure(ctx->oseg, std::max<size_t>(ns, 1) * max_seg * BFJ_PIVOTS * 4));
+    RC(ensure(ctx->pivots, std::max<size_t>(ns, 1) * BFJ_PIVOTS * BFJ_MAXDIM * 2));
+    RC(ensure(ctx->pivot_norm, std::max<size_t>(ns, 1) * BFJ_PIVOTS * 4));
     HC(hipMemcpyAsync(ctx->sets.p, p.sets.data(), ns * sizeof(SetDesc), hipMemcpyHostToDevice, ctx->stream));
     if (n_items && (ctx->items_staged != p.items.get() || ctx->items_staged_buf != ctx->items.p)) {
         ctx->items_staged = p.items.get();
@@ -499,6 +511,15 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.pairs = (uint2 *)ctx->pairs.p;
     a.pair_count = (unsigned long long *)((uint32_t *)ctx->counters.p + 4);
     a.pair_capacity = ctx->pair_capacity;
+    int max_nblk = 1;
+    for (int32_t v : p.set_nblk) max_nblk = std::max(max_nblk, (int)v);
+    a.perm = (uint32_t *)ctx->perm.p;
+    a.label = (uint8_t *)ctx->label.p;
+    a.ohist = (uint32_t *)ctx->ohist.p;
+    a.oseg = (uint32_t *)ctx->oseg.p;
+    a.oseg_stride = (max_nblk + BFJ_ORD_SEG - 1) / BFJ_ORD_SEG;
+    a.pivots = (_Float16 *)ctx->pivots.p;
+    a.pivot_norm = (float *)ctx->pivot_norm.p;
     return a;
 }
 
@@ -714,7 +735,8 @@ int bfj_destroy(bfj_ctx *c)
     (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
                       &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
-                      &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux};
+                      &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux,
+                      &c->perm, &c->label, &c->ohist, &c->oseg, &c->pivots, &c->pivot_norm};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -1129,6 +1151,13 @@ int bfj_tile_rows_options(int32_t dtype, int32_t d, int32_t *options, int32_t ma
             ++k;
         }
     *n_options = k;
+    return BFJ_OK;
+}
+
+int bfj_set_reorder(bfj_ctx *ctx, int32_t enable)
+{
+    if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
+    ctx->reorder = enable ? 1 : 0;
     return BFJ_OK;
 }
 

@@ -53,6 +53,7 @@ struct SetDesc {
     int32_t set_id;
     int32_t sym;       // every row is a query: each unordered pair is evaluated once
     int32_t al16;      // x and ld keep every row 16-byte aligned
+    int32_t reorder;   // rows packed in a locality order (perm); 0: position = row
     double T[BFJ_MAXEPS]; // input-space thresholds, ascending: hit iff d2 <= T
     int8_t eperm[BFJ_MAXEPS]; // T[k] belongs to the caller's eps[eperm[k]]
 };
@@ -166,7 +167,24 @@ struct PassArgs {
     uint2 *pairs;                      // (i, j), i < j
     unsigned long long *pair_count;    // pairs found (may exceed pair_capacity)
     unsigned long long pair_capacity;
+    // Row order (order.hip): position -> dataset-local row, per row_off + position.
+    uint32_t *perm;
+    uint8_t *label;                    // nearest pivot per row (input order)
+    uint32_t *ohist;                   // [prep block][BFJ_PIVOTS] counts, then offsets
+    uint32_t *oseg;                    // [set][oseg_stride][BFJ_PIVOTS] segment offsets
+    int32_t oseg_stride;               // scan segments of the largest dataset
+    _Float16 *pivots;                  // [set][BFJ_PIVOTS][BFJ_MAXDIM]
+    float *pivot_norm;                 // [set][BFJ_PIVOTS]
 };
+
+#define BFJ_PIVOTS 128   // pivot rows of the locality order
+#define BFJ_ORD_SEG 256  // prep blocks per scan segment of the locality order
+
+// Dataset-local row held at position i (the identity unless the set is reordered).
+__device__ inline uint32_t bfj_row_at(const PassArgs &a, const SetDesc &s, int64_t i)
+{
+    return a.perm[s.row_off + i];
+}
 
 #define BFJ_MAXDIM 256
 #define BFJ_PREP_ROWS 64   // rows per block in row-parallel prep kernels
@@ -177,3 +195,4 @@ void bfj_launch_classify(const PassArgs &a, hipStream_t s);
 void bfj_launch_refine(const PassArgs &a, hipStream_t s);
 void bfj_launch_finalize(const PassArgs &a, hipStream_t s);
 void bfj_launch_all_pairs(uint2 *pairs, int64_t n, hipStream_t s);
+void bfj_launch_order(const PassArgs &a, hipStream_t s);

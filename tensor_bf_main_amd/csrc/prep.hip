@@ -205,6 +205,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_quantize(PassArgs a)
         const int64_t i = a.blk_row0[b] + wave * (BFJ_PREP_ROWS / 4) + rr;
         if (i >= s.npad) break;
         const bool real = i < s.n;
+        const int64_t src = real ? (int64_t)bfj_row_at(a, s, i) : 0; // the row packed at position i
         // Centred, scaled row values for this lane's k = lane + 64 t.
         double y[BFJ_MAXDIM / BFJ_WAVE];
         double rmax = 0.0;
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_quantize(PassArgs a)
         for (int t = 0; t < BFJ_MAXDIM / BFJ_WAVE; ++t) {
             const int k = lane + BFJ_WAVE * t;
             double v = 0.0;
-            if (real && k < s.d) v = (s.x[i * s.ld + k] - mu[k]) * inv_sigma;
+            if (real && k < s.d) v = (s.x[src * s.ld + k] - mu[k]) * inv_sigma;
             y[t] = v;
             rmax = fmax(rmax, fabs(v));
         }
@@ -426,6 +427,7 @@ void bfj_launch_prep(const PassArgs &a, hipStream_t st)
     hipLaunchKernelGGL(k_colmean, dim3(a.n_sets, BFJ_MAXDIM), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_maxabs, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_sigma, dim3((a.n_sets + 255) / 256), dim3(256), 0, st, a);
+    bfj_launch_order(a, st);
     hipLaunchKernelGGL(k_quantize, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_thresholds, dim3(a.n_row_blocks), dim3(BFJ_PREP_ROWS), 0, st, a);
 }

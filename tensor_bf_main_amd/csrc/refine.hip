@@ -42,10 +42,22 @@ __device__ inline void add_bucket(const PassArgs &a, uint32_t key, const SetDesc
     }
 }
 
-// Pair-list mode: append (i, j) for the lanes with `take` (one atomic per wavefront).
-// Pairs past the capacity are counted, not written.  All lanes take part.
-__device__ inline void append_pairs(const PassArgs &a, bool take, uint32_t i, uint32_t j, int lane)
+// Pair-list mode: append the pair at packed positions (i, j) for the lanes with
+// `take` (one atomic per wavefront), as dataset rows, smaller first.  Pairs past the
+// capacity are counted, not written.  All lanes take part.
+__device__ inline void append_pairs(const PassArgs &a, const SetDesc &s, bool take, uint32_t pi, uint32_t pj,
+                                    int lane)
 {
+    uint32_t i = 0, j = 0;
+    if (take) {
+        i = bfj_row_at(a, s, pi);
+        j = bfj_row_at(a, s, pj);
+        if (i > j) {
+            const uint32_t t = i;
+            i = j;
+            j = t;
+        }
+    }
     const unsigned long long bal = __ballot(take);
     if (!bal) return;
     unsigned long long base = 0;
@@ -134,7 +146,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
             const bool band = (d.band_i | d.band_j) != 0u;
             add_bucket(a, d.c.row, s, d.i, (!band && d.in_i) ? __builtin_ctz(d.in_i) : -1, lane);
             add_bucket(a, (uint32_t)s.row_off + d.j, s, d.j, (!band && d.in_j) ? __builtin_ctz(d.in_j) : -1, lane);
-            if (a.emit_pairs) append_pairs(a, !band && (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane); // i <= j
+            if (a.emit_pairs) append_pairs(a, s, !band && (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane);
             const uint32_t nb = (uint32_t)__popcll(__ballot(band));
             if (nb) band_groups |= 1u << g;
             n_band += nb;
@@ -217,8 +229,9 @@ __device__ inline void refine_one(const PassArgs &a, const BandEntry &be, bool l
     const int si = (int)(be.meta >> 16);
     const SetDesc &s = a.sets[si];
     const int d = live ? s.d : 0;
-    const bfj_gptr p = (bfj_gptr)(s.x + (int64_t)be.i * s.ld);
-    const bfj_gptr q = (bfj_gptr)(s.x + (int64_t)be.j * s.ld);
+    // be.i / be.j are packed positions; the FP64 rows are the dataset's rows held there.
+    const bfj_gptr p = (bfj_gptr)(s.x + (int64_t)(live ? bfj_row_at(a, s, be.i) : 0) * s.ld);
+    const bfj_gptr q = (bfj_gptr)(s.x + (int64_t)(live ? bfj_row_at(a, s, be.j) : 0) * s.ld);
     // Each lane runs the sequential FMA chain of its own pair.  Whole chunks go
     // through two register buffers, the next chunk's loads in flight while the
     // current one is consumed; the last d % RK values are loaded one by one.
@@ -258,7 +271,7 @@ __device__ inline void refine_one(const PassArgs &a, const BandEntry &be, bool l
             }
     add_bucket(a, (uint32_t)s.row_off + be.i, s, be.i, k, lane);
     add_bucket(a, (uint32_t)s.row_off + be.j, s, be.j, (be.meta & 0x8000u) ? k : -1, lane);
-    if (a.emit_pairs) append_pairs(a, k == 0 && be.i != be.j, be.i, be.j, lane);
+    if (a.emit_pairs) append_pairs(a, s, k == 0 && be.i != be.j, be.i, be.j, lane);
     if (live) {
         const double sigma = a.scal[si].sigma;
         const double approx = sigma * sigma * (a.he[s.row_off + be.i].x - 2.0 * (double)be.z);
@@ -342,8 +355,9 @@ __global__ void k_finalize(PassArgs a)
     const int b = blockIdx.x;
     const int si = a.blk_set[b];
     const SetDesc &s = a.sets[si];
-    const int64_t i = a.blk_row0[b] + threadIdx.x;
-    const bool query = i < s.n && i >= s.row_begin && i < s.row_end;
+    const int64_t i = a.blk_row0[b] + threadIdx.x; // packed position
+    const int64_t oi = i < s.n ? (int64_t)bfj_row_at(a, s, i) : i; // the dataset row held there
+    const bool query = i < s.n && oi >= s.row_begin && oi < s.row_end;
     const int64_t rows = s.row_end - s.row_begin;
     unsigned long long run = 0; // buckets 0..e: pairs within the e-th smallest eps
     for (int e = 0; e < s.n_eps; ++e) {
@@ -357,7 +371,7 @@ __global__ void k_finalize(PassArgs a)
             // A symmetric sweep streams no self pairs: d2(i, i) == 0 is within every eps >= 0.
             if (!all && s.sym && s.T[e] >= 0.0) c += 1;
             const long long self = (s.T[e] >= 0.0) ? 1 : 0; // d2(i, i) == 0 exactly
-            if (s.out) s.out[(int64_t)s.eperm[e] * rows + (i - s.row_begin)] = (long long)c - self;
+            if (s.out) s.out[(int64_t)s.eperm[e] * rows + (oi - s.row_begin)] = (long long)c - self;
         }
         red[e][threadIdx.x] = c;
     }

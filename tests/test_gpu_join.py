@@ -294,3 +294,26 @@ def test_pair_list_edges(ctx, dtype):
     rc = ctx.lib.bfj_self_join_pairs(ctx._h, xc.ctypes.data, n, xc.shape[1], xc.shape[1], 1e6, bfj.DTYPES[dtype],
                                      buf.ctypes.data, 5, ctypes.byref(found), ctypes.byref(tot), None)
     assert rc == bfj.BFJ_ERR_CAPACITY and found.value == n * (n - 1) // 2
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("n,d,kind", [(512, 18, "gmm"), (3000, 128, "gmm"), (2500, 33, "uniform"), (4097, 200, "gmm")])
+def test_locality_order_gives_the_same_results(ctx, dtype, n, d, kind):
+    """Whole-dataset passes pack rows grouped by nearest pivot (order.hip); per-point
+    counts, totals and pair lists must not depend on it, and must match the oracle."""
+    x = wl.gmm(n, d, 8, 0.05, n + d) if kind == "gmm" else wl.uniform(n, d, n + d)
+    eps = wl.eps_quantile(x, [0.002, 0.02], 200, 5)
+    try:
+        ctx.set_reorder(False)
+        plain = ctx.self_join(x, eps, dtype=dtype)
+        plain_pairs, _, _ = ctx.self_join_pairs(x, eps[0], dtype=dtype)
+    finally:
+        ctx.set_reorder(True)
+    res = _check(ctx, x, eps, dtype)
+    np.testing.assert_array_equal(res.counts, plain.counts)
+    np.testing.assert_array_equal(res.totals, plain.totals)
+    pairs, total, _ = ctx.self_join_pairs(x, eps[0], dtype=dtype)
+    assert total == int(res.totals[0])
+    assert np.all(pairs[:, 0] < pairs[:, 1])
+    key = lambda p: np.sort(p[:, 0].astype(np.int64) * n + p[:, 1])
+    np.testing.assert_array_equal(key(pairs), key(plain_pairs))
