@@ -137,9 +137,14 @@ static int lds_for(int dtype, int kc, int rows, int n_eps)
 // across blocks matters more than strip width up to there), then the widest strip.
 // Measured on MI355X: FP16/BF16/FP64 peak at 3 blocks per CU with wider strips, FP8
 // (twice the MFMA rate, so more issue and LDS work per flop) at 4.
-static int heuristic_rows(int dtype, int kc, int n_eps)
+// Tile rows R: the widest strip among those with the most resident blocks per CU,
+// counted up to a cap.  Measured: FP8 wants 4 blocks per CU and FP16/BF16 3 in input
+// order (R = 64 at d = 128); in the locality order hot tiles are rare, and 2 blocks of
+// wider strips win (FP16 d = 128: R = 128, 76 ms against 78 ms at R = 64).
+static int heuristic_rows(int dtype, int kc, int n_eps, bool reordered)
 {
-    const int cap = dtype == BFJ_FP8 ? 4 : 3;
+    const bool low = dtype == BFJ_FP16 || dtype == BFJ_BF16;
+    const int cap = dtype == BFJ_FP8 ? 4 : (low && reordered ? 2 : 3);
     int best = 0, best_occ = -1;
     for (int r : kRowOptions)
         if (bfj_join_has(dtype, kc, r)) {
@@ -344,10 +349,14 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         memset(&g, 0, sizeof g);
         g.kc = kc;
         int n_eps_max = 1;
+        bool reordered = true;
         for (auto &s : p.sets)
-            if (s.kc == kc) n_eps_max = std::max(n_eps_max, (int)s.n_eps);
+            if (s.kc == kc) {
+                n_eps_max = std::max(n_eps_max, (int)s.n_eps);
+                reordered = reordered && s.reorder;
+            }
         int rows = force_rows ? force_rows : ctx->tile_rows[dtype][kc];
-        if (!rows || !bfj_join_has(dtype, kc, rows)) rows = heuristic_rows(dtype, kc, n_eps_max);
+        if (!rows || !bfj_join_has(dtype, kc, rows)) rows = heuristic_rows(dtype, kc, n_eps_max, reordered);
         if (!bfj_join_has(dtype, kc, rows)) return fail(BFJ_ERR_UNSUPPORTED, "no tile for dtype %d kc %d", dtype, kc);
         g.rows = rows;
         g.lds = lds_for(dtype, kc, rows, n_eps_max);

@@ -178,7 +178,7 @@ struct PassArgs {
 };
 
 #define BFJ_PIVOTS 128   // pivot rows of the locality order
-#define BFJ_ORD_SEG 256  // prep blocks per scan segment of the locality order
+#define BFJ_ORD_SEG 64   // prep blocks per scan segment of the locality order
 
 // Dataset-local row held at position i (the identity unless the set is reordered).
 __device__ inline uint32_t bfj_row_at(const PassArgs &a, const SetDesc &s, int64_t i)

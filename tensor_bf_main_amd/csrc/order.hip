@@ -90,13 +90,25 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_order_label(PassArgs a)
             *(ord_h8 *)(pch + p * 32 + q * 8) = *(const ord_h8 *)(pv + p * BFJ_MAXDIM + c * 32 + q * 8);
         }
         __syncthreads();
-        // A fragment: row r, columns 32c + 8 (lane / 16) + j.
+        // A fragment: row r, columns 32c + 8 (lane / 16) + j; 16-byte loads when the
+        // row's 8 values are whole and aligned.
         ord_h8 av;
+        const int k0 = c * 32 + 8 * (lane >> 4);
+        if (real && s.al16 && k0 + 8 <= s.d) {
+            const double2 *src = reinterpret_cast<const double2 *>(s.x + r * s.ld + k0);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int k = c * 32 + 8 * (lane >> 4) + j;
-            const double v = (real && k < s.d) ? (s.x[r * s.ld + k] - mu[k]) * inv_sigma : 0.0;
-            av[j] = (_Float16)v;
+            for (int h = 0; h < 4; ++h) {
+                const double2 w2 = src[h];
+                av[2 * h] = (_Float16)((w2.x - mu[k0 + 2 * h]) * inv_sigma);
+                av[2 * h + 1] = (_Float16)((w2.y - mu[k0 + 2 * h + 1]) * inv_sigma);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + j;
+                const double v = (real && k < s.d) ? (s.x[r * s.ld + k] - mu[k]) * inv_sigma : 0.0;
+                av[j] = (_Float16)v;
+            }
         }
 #pragma unroll
         for (int t = 0; t < BFJ_PIVOTS / 16; ++t) {
