@@ -298,6 +298,117 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_quantize(PassArgs a)
     }
 }
 
+// FP16 / BF16: one group of L lanes per row, each lane 8 consecutive columns, which
+// are exactly one 16-byte unit of the packed fragments (16x16x32 layout: K-chunk
+// k/32, lane-quarter (k%32)/8), so a row is read with 16-byte loads and written
+// with one 16-byte store per lane; 64/L rows per wavefront at a time.  The row's
+// sums are then reduced over its L lanes only.  Same outputs as k_quantize.
+template <bool BF>
+__global__ __launch_bounds__(BFJ_BLOCK) void k_quantize16(PassArgs a)
+{
+    __shared__ double redq[BFJ_BLOCK / BFJ_WAVE], redn[BFJ_BLOCK / BFJ_WAVE];
+    const int b = blockIdx.x;
+    const int si = a.blk_set[b];
+    const SetDesc &s = a.sets[si];
+    const double *mu = a.mu + (int64_t)si * BFJ_MAXDIM;
+    const double inv_sigma = 1.0 / a.scal[si].sigma; // exact: sigma is a power of two
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int units = s.kc * 4;                  // 8-column units of a padded row
+    int L = 4;
+    while (L < units) L <<= 1;                   // lanes per row: a power of two <= 64
+    const int rpp = BFJ_WAVE / L;                // rows per pass
+    const int m = lane & (L - 1), lr = lane / L; // this lane's unit and row of the pass
+    const int k0 = 8 * m;
+    double qmax = 0.0, nmax = 0.0;
+    const int64_t wrow0 = a.blk_row0[b] + wave * (BFJ_PREP_ROWS / 4);
+    for (int rr = 0; rr < BFJ_PREP_ROWS / 4; rr += rpp) {
+        const int64_t i = wrow0 + rr + lr;
+        const bool in = rr + lr < BFJ_PREP_ROWS / 4 && i < s.npad && m < units;
+        const bool real = in && i < s.n;
+        const int64_t src = real ? (int64_t)bfj_row_at(a, s, i) : 0;
+        double y[8];
+        if (real && s.al16 && k0 + 8 <= s.d) {
+            const double2 *p2 = reinterpret_cast<const double2 *>(s.x + src * s.ld + k0);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const double2 w2 = p2[h];
+                y[2 * h] = (w2.x - mu[k0 + 2 * h]) * inv_sigma;
+                y[2 * h + 1] = (w2.y - mu[k0 + 2 * h + 1]) * inv_sigma;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + j;
+                y[j] = (real && k < s.d) ? (s.x[src * s.ld + k] - mu[k]) * inv_sigma : 0.0;
+            }
+        }
+        double sq_e = 0.0, sq_n = 0.0, sq_y = 0.0;
+        uint32_t w[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            uint16_t bits[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const double v = y[2 * h + u];
+                double xh;
+                if (BF) {
+                    const __bf16 q = (__bf16)(float)v;
+                    xh = (double)(float)q;
+                    bits[u] = __builtin_bit_cast(uint16_t, q);
+                } else {
+                    const _Float16 q = (_Float16)v;
+                    xh = (double)q;
+                    bits[u] = __builtin_bit_cast(uint16_t, q);
+                }
+                const double de = v - xh;
+                sq_e = fma(de, de, sq_e);
+                sq_n = fma(xh, xh, sq_n);
+                sq_y = fma(v, v, sq_y);
+            }
+            w[h] = (uint32_t)bits[0] | ((uint32_t)bits[1] << 16);
+        }
+        if (in) {
+            const int c = m >> 2, g = m & 3;
+            uint4 *dst = reinterpret_cast<uint4 *>(a.pack + s.pack_off) +
+                         ((i / 16) * s.kc + c) * BFJ_WAVE + (i % 16) + 16 * g;
+            *dst = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        for (int o = L >> 1; o > 0; o >>= 1) {
+            sq_e += __shfl_xor(sq_e, o);
+            sq_n += __shfl_xor(sq_n, o);
+            sq_y += __shfl_xor(sq_y, o);
+        }
+        if (m == 0 && rr + lr < BFJ_PREP_ROWS / 4 && i < s.npad) {
+            const int64_t row = s.row_off + i;
+            // Radius: measured cast error, plus the FP64 rounding of (x - mu), inflated
+            // to cover the FP64 evaluation of the sums (as in k_quantize).
+            const double q = real ? (sqrt(sq_e) + 2.3e-16 * sqrt(sq_y)) * (1.0 + 1e-9) + 1e-300 : 0.0;
+            a.qerr[row] = q;
+            a.nrm[row] = sq_n;
+            ((float *)a.nq)[row] = real ? 0.5f * (float)sq_n : 1e30f;
+            if (real) {
+                qmax = fmax(qmax, q);
+                nmax = fmax(nmax, sq_n);
+            }
+        }
+    }
+    qmax = wave_max(qmax);
+    nmax = wave_max(nmax);
+    if (lane == 0) {
+        redq[wave] = qmax;
+        redn[wave] = nmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w2 = 1; w2 < BFJ_BLOCK / BFJ_WAVE; ++w2) {
+            qmax = fmax(qmax, redq[w2]);
+            nmax = fmax(nmax, redn[w2]);
+        }
+        atomic_max_nonneg(&a.scal[si].qmax_bits, qmax);
+        atomic_max_nonneg(&a.scal[si].nmax_bits, nmax);
+    }
+}
+
 // ---- thresholds -----------------------------------------------------------------
 __device__ inline float f32_up(double v)
 {
@@ -428,6 +539,8 @@ void bfj_launch_prep(const PassArgs &a, hipStream_t st)
     hipLaunchKernelGGL(k_maxabs, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_sigma, dim3((a.n_sets + 255) / 256), dim3(256), 0, st, a);
     bfj_launch_order(a, st);
-    hipLaunchKernelGGL(k_quantize, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
+    if (a.dtype == BFJ_FP16) hipLaunchKernelGGL(k_quantize16<false>, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
+    else if (a.dtype == BFJ_BF16) hipLaunchKernelGGL(k_quantize16<true>, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
+    else hipLaunchKernelGGL(k_quantize, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_thresholds, dim3(a.n_row_blocks), dim3(BFJ_PREP_ROWS), 0, st, a);
 }
