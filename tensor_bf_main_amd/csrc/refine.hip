@@ -42,6 +42,93 @@ __device__ inline void add_bucket(const PassArgs &a, uint32_t key, const SetDesc
     }
 }
 
+// Per-wavefront count table in LDS: cnt index -> pending count, open addressing with
+// linear probing.  Counts of keys that repeat over many records (the rows of a strip,
+// the columns of a dense tile) add up here and reach cnt with one global atomic per
+// key per flush instead of one per key per 64 records.  Keys are cnt indices (used
+// when the whole cnt array is indexable in 32 bits; the caller checks).  A key that
+// finds no slot within BFJ_CTAB_PROBES probes goes straight to cnt.
+#ifndef BFJ_CTAB
+#define BFJ_CTAB 512 // slots per wavefront (a power of two)
+#endif
+#define BFJ_CTAB_PROBES 8
+#define BFJ_CTAB_EMPTY 0xffffffffu
+struct CountTable {
+    uint32_t key[BFJ_CTAB];
+    uint32_t cnt[BFJ_CTAB];
+    uint16_t used[BFJ_CTAB]; // slots claimed since the last flush, in claim order
+    uint32_t n_used;
+};
+
+__device__ inline void ctab_clear(CountTable &t, int lane)
+{
+    for (int h = lane; h < BFJ_CTAB; h += BFJ_WAVE) {
+        t.key[h] = BFJ_CTAB_EMPTY;
+        t.cnt[h] = 0;
+    }
+    if (lane == 0) t.n_used = 0;
+    __builtin_amdgcn_wave_barrier();
+}
+
+// cnt[idx] += 1 for the lanes with `has`, through the table.
+__device__ inline void ctab_add(const PassArgs &a, CountTable &t, uint32_t idx, bool has)
+{
+    if (!has) return;
+    uint32_t h = (idx * 2654435761u) >> (32 - __builtin_ctz(BFJ_CTAB));
+    for (int p = 0; p < BFJ_CTAB_PROBES; ++p) {
+        uint32_t old = BFJ_CTAB_EMPTY;
+        __hip_atomic_compare_exchange_strong(&t.key[h], &old, idx, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == BFJ_CTAB_EMPTY)
+            t.used[__hip_atomic_fetch_add(&t.n_used, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)] =
+                (uint16_t)h;
+        if (old == BFJ_CTAB_EMPTY || old == idx) {
+            // Workgroup scope: an LDS atomic the compiler cannot merge with the
+            // global one below into a flat atomic.
+            __hip_atomic_fetch_add(&t.cnt[h], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return;
+        }
+        h = (h + 1) & (BFJ_CTAB - 1);
+    }
+    atomicAdd(&a.cnt[idx], 1u);
+}
+
+// Moves the pending counts to cnt (one atomic per claimed slot) and empties the
+// table.  All lanes take part.
+__device__ inline void ctab_flush(const PassArgs &a, CountTable &t, int lane)
+{
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t n = __hip_atomic_load(&t.n_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    for (uint32_t q = lane; q < n; q += BFJ_WAVE) {
+        const uint32_t h = t.used[q];
+        atomicAdd(&a.cnt[t.key[h]], t.cnt[h]);
+        t.key[h] = BFJ_CTAB_EMPTY;
+        t.cnt[h] = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) t.n_used = 0;
+    __builtin_amdgcn_wave_barrier();
+}
+
+// cnt[bi][i] += 1 and cnt[bj][j] += 1 for rows i, j of dataset s (bucket -1: no
+// count), through the table when use_tab holds (uniform), else by the vote loop.
+// All lanes take part.
+__device__ inline void count_rows(const PassArgs &a, CountTable &tab, bool use_tab, const SetDesc &s, uint32_t i,
+                                  int bi, uint32_t j, int bj, int lane)
+{
+    if (use_tab) {
+        // Room for this call's keys (at most 2 per lane) at under 5/8 load.
+        if (__hip_atomic_load(&tab.n_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) >
+            (uint32_t)(BFJ_CTAB * 5 / 8 - 2 * BFJ_WAVE))
+            ctab_flush(a, tab, lane);
+        ctab_add(a, tab, (uint32_t)(s.cnt_off + (int64_t)bi * s.npad + i), bi >= 0);
+        ctab_add(a, tab, (uint32_t)(s.cnt_off + (int64_t)bj * s.npad + j), bj >= 0);
+    } else {
+        add_bucket(a, (uint32_t)s.row_off + i, s, i, bi, lane);
+        add_bucket(a, (uint32_t)s.row_off + j, s, j, bj, lane);
+    }
+}
+
 // Pair-list mode: append the pair at packed positions (i, j) for the lanes with
 // `take` (one atomic per wavefront), as dataset rows, smaller first.  Pairs past the
 // capacity are counted, not written.  All lanes take part.
@@ -81,8 +168,11 @@ __device__ inline void decide(const SetScalars &sc, uint32_t band_mask, int n_ep
         const double slo = sc.slo[e], shi = sc.shi[e];
         const double L = slo > r ? (slo - r) * (slo - r) * (1.0 - 0x1p-49) : -INFINITY;
         const double H = (shi + r) * (shi + r) * (1.0 + 0x1p-49);
-        if (hi <= L) in |= 1u << e;
-        else if (lo <= H) band |= 1u << e;
+        // Flags by arithmetic, not through a selected reference: that form puts in
+        // and band in scratch memory.
+        const bool cin = hi <= L;
+        in |= (uint32_t)cin << e;
+        band |= (uint32_t)(!cin && lo <= H) << e;
     }
 }
 
@@ -129,6 +219,10 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
+    __shared__ CountTable tabs[BFJ_BLOCK / BFJ_WAVE];
+    CountTable &tab = tabs[threadIdx.x >> 6];
+    const bool use_tab = a.cnt_len < (int64_t)BFJ_CTAB_EMPTY;
+    if (use_tab) ctab_clear(tab, lane);
     const uint32_t n_chunks = min(*a.chunk_count, a.chunk_capacity);
     const uint32_t wave_id = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
     const uint32_t n_waves = gridDim.x * (BFJ_BLOCK / BFJ_WAVE);
@@ -144,8 +238,9 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
             // A record with an eps in the band is decided whole by the FP64 refine;
             // otherwise it counts once, under the first eps that certainly holds it.
             const bool band = (d.band_i | d.band_j) != 0u;
-            add_bucket(a, d.c.row, s, d.i, (!band && d.in_i) ? __builtin_ctz(d.in_i) : -1, lane);
-            add_bucket(a, (uint32_t)s.row_off + d.j, s, d.j, (!band && d.in_j) ? __builtin_ctz(d.in_j) : -1, lane);
+            const int bi = (!band && d.in_i) ? __builtin_ctz(d.in_i) : -1;
+            const int bj = (!band && d.in_j) ? __builtin_ctz(d.in_j) : -1;
+            count_rows(a, tab, use_tab, s, d.i, bi, d.j, bj, lane);
             if (a.emit_pairs) append_pairs(a, s, !band && (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane);
             const uint32_t nb = (uint32_t)__popcll(__ballot(band));
             if (nb) band_groups |= 1u << g;
@@ -178,6 +273,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
             base += (uint32_t)__popcll(bb);
         }
     }
+    if (use_tab) ctab_flush(a, tab, lane);
 }
 
 void bfj_launch_classify(const PassArgs &a, hipStream_t st)
@@ -223,8 +319,8 @@ __device__ inline double fma_chunk(const double (&u)[RK], const double (&v)[RK],
 // bucket of the first eps (ascending) whose threshold holds it, counted for its row
 // and (symmetric record) for the other row, and the approximation error for the
 // statistics.  All lanes take part.
-__device__ inline void refine_one(const PassArgs &a, const BandEntry &be, bool live, int lane, int &err_set,
-                                  double &err_run)
+__device__ inline void refine_one(const PassArgs &a, const BandEntry &be, bool live, int lane, CountTable &tab,
+                                  bool use_tab, int &err_set, double &err_run)
 {
     const int si = (int)(be.meta >> 16);
     const SetDesc &s = a.sets[si];
@@ -269,8 +365,7 @@ __device__ inline void refine_one(const PassArgs &a, const BandEntry &be, bool l
                 k = e;
                 break;
             }
-    add_bucket(a, (uint32_t)s.row_off + be.i, s, be.i, k, lane);
-    add_bucket(a, (uint32_t)s.row_off + be.j, s, be.j, (be.meta & 0x8000u) ? k : -1, lane);
+    count_rows(a, tab, use_tab, s, be.i, k, be.j, (be.meta & 0x8000u) ? k : -1, lane);
     if (a.emit_pairs) append_pairs(a, s, k == 0 && be.i != be.j, be.i, be.j, lane);
     if (live) {
         const double sigma = a.scal[si].sigma;
@@ -304,25 +399,38 @@ __device__ inline void refine_flush(const PassArgs &a, int err_set, double err_r
     }
 }
 
+// Each wavefront takes runs of BFJ_REFINE_RUN consecutive records (classify writes a
+// chunk's records together, so the rows of a run repeat and their counts meet in
+// the wavefront's count table), the runs dealt round-robin over the wavefronts.
+#define BFJ_REFINE_RUN 512
 __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t cnt = *a.band_count;
-    const uint32_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
-    const uint32_t stride = gridDim.x * BFJ_BLOCK;
+    const uint64_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
+    const uint64_t wave = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
+    const uint64_t step = (uint64_t)gridDim.x * (BFJ_BLOCK / BFJ_WAVE) * BFJ_REFINE_RUN;
+    __shared__ CountTable tabs[BFJ_BLOCK / BFJ_WAVE];
+    CountTable &tab = tabs[threadIdx.x >> 6];
+    const bool use_tab = a.cnt_len < (int64_t)BFJ_CTAB_EMPTY;
+    if (use_tab) ctab_clear(tab, lane);
     int err_set = -1;
     double err_run = -1.0;
     // base is uniform over the wavefront, so every lane takes part in the votes.
-    for (uint32_t base = blockIdx.x * BFJ_BLOCK + (threadIdx.x & ~63u); base < m; base += stride) {
-        const uint32_t t = base + lane;
-        const bool live = t < m;
-        BandEntry be;
-        be.i = be.j = 0;
-        be.z = 0.f;
-        be.meta = 0;
-        if (live) be = a.band[t];
-        refine_one(a, be, live, lane, err_set, err_run);
+    for (uint64_t run = wave * BFJ_REFINE_RUN; run < m; run += step) {
+        const uint64_t end = run + BFJ_REFINE_RUN < m ? run + BFJ_REFINE_RUN : m;
+        for (uint64_t base = run; base < end; base += BFJ_WAVE) {
+            const uint64_t t = base + lane;
+            const bool live = t < end;
+            BandEntry be;
+            be.i = be.j = 0;
+            be.z = 0.f;
+            be.meta = 0;
+            if (live) be = a.band[t];
+            refine_one(a, be, live, lane, tab, use_tab, err_set, err_run);
+        }
     }
+    if (use_tab) ctab_flush(a, tab, lane);
     refine_flush(a, err_set, err_run, lane);
 }
 
