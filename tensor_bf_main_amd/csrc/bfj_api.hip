@@ -379,7 +379,7 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         // in flight on an XCD sweep the same panel and its tiles are fetched into that
         // XCD's L2 once instead of once per strip.  Datasets largest first.
         const int64_t tile_bytes = (int64_t)kc * BFJ_WAVE * FB;
-        const int panel = (int)std::max<int64_t>(32, (1 << 20) / tile_bytes / 16 * 16);
+        const int panel = (int)std::max<int64_t>(32, (int64_t)BFJ_PANEL_BYTES / tile_bytes / 16 * 16);
         const int strip = BFJ_JWAVES * rows / 16;
         std::vector<size_t> order;
         for (size_t si = 0; si < p.sets.size(); ++si)

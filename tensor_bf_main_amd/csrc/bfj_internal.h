@@ -101,6 +101,9 @@ struct Candidate {
 #define BFJ_COL_SYM 0x80000000u
 
 #define BFJ_CHUNK 1024 // candidate records per chunk
+#ifndef BFJ_PANEL_BYTES
+#define BFJ_PANEL_BYTES (1 << 20) // candidate fragments per join work item
+#endif
 
 struct JoinArgs {
     const SetDesc *sets;

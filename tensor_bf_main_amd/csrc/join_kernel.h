@@ -85,7 +85,10 @@ struct TrF64 {
 __device__ inline int bfj_lane() { return __lane_id(); }
 
 #ifndef BFJ_SLOT_BYTES
-#define BFJ_SLOT_BYTES 8192 // candidate fragment bytes per ring slot
+#define BFJ_SLOT_BYTES 16384 // at most this many candidate fragment bytes per ring slot
+#endif
+#ifndef BFJ_SLOT_MFMA
+#define BFJ_SLOT_MFMA 128 // MFMAs per wavefront per ring slot to aim for
 #endif
 #ifndef BFJ_DIRECT_B
 #define BFJ_DIRECT_B 0 // 1: candidate fragments read straight into VGPRs, no LDS ring
@@ -100,12 +103,19 @@ struct JoinLds {
     typedef typename Tr::val_t val_t;
     static constexpr int FB = (int)sizeof(typename Tr::frag_t);
     static constexpr int TILE = KC * BFJ_WAVE * FB; // candidate fragments of one 16-row block
-    // A ring slot holds CPT consecutive candidate blocks (about 8 KiB of fragments):
-    // one wait, one barrier and one set of loads per slot instead of per block.
-    static constexpr int CPT_RAW = BFJ_SLOT_BYTES / TILE;
+    // A ring slot holds CPT consecutive candidate blocks: one wait, one barrier and
+    // one set of loads per slot instead of per block.  Enough blocks that each
+    // wavefront issues about BFJ_SLOT_MFMA MFMAs (R/16 x KC per block) between
+    // barriers, within BFJ_SLOT_BYTES.  Measured at FP16 R = 128, d = 128: 4 blocks
+    // (128 MFMAs) per slot take 1.4 % off the join against 2.
+    static constexpr int CPT_MFMA = BFJ_SLOT_MFMA / ((R / 16) * KC) < 1 ? 1 : BFJ_SLOT_MFMA / ((R / 16) * KC);
+    static constexpr int CPT_BYTES = BFJ_SLOT_BYTES / TILE;
+    static constexpr int CPT_RAW = CPT_MFMA < CPT_BYTES ? CPT_MFMA : CPT_BYTES;
     // (FP64 slots hold at most 2 blocks, so a slot's norms fit one 64-lane load.)
     static constexpr int CPT_MAX = (int)sizeof(val_t) == 8 ? 2 : 4;
-    static constexpr int CPT = CPT_RAW < 1 ? 1 : (CPT_RAW > CPT_MAX ? CPT_MAX : CPT_RAW);
+    // 1, 2 or 4: panels and strips are multiples of 4 blocks, so items are whole slots.
+    static constexpr int CPT_CLAMP = CPT_RAW < 1 ? 1 : (CPT_RAW > CPT_MAX ? CPT_MAX : CPT_RAW);
+    static constexpr int CPT = CPT_CLAMP >= 4 ? 4 : (CPT_CLAMP >= 2 ? 2 : 1);
     static constexpr int UNITS = CPT * TILE / 16; // 16-byte units of a slot
     // Direct-to-LDS loads per wavefront per slot: the fragment area is padded to a
     // whole number of block-wide 16-byte loads so every wavefront issues the same
