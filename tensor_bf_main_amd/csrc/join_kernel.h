@@ -281,8 +281,21 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         else __builtin_amdgcn_s_waitcnt((N2 & 15) | ((N2 >> 4) << 14) | (7 << 4) | (15 << 8));
     };
 
-    // One candidate block cb against this wavefront's rows.
-    auto block = [&](int cb, const frag_t (&B)[KC], val_t nj, int SB) {
+#ifndef BFJ_PIPE_MFMA
+#define BFJ_PIPE_MFMA 16
+#endif
+    // Read-ahead of the next block's fragments pays where a block has enough MFMA
+    // work to cover an LDS read.  Measured: -1 % on the FP16 join at d = 128 (32
+    // MFMAs per block), -3 % at BF16 d = 256, -2 % for FP8 at 4 MFMAs of K = 128 per
+    // block, but +4 % on a batch of small-d FP16 tiles with 4-8 MFMAs per block,
+    // where it costs more in registers than it hides.
+    constexpr bool PIPE = RB * KC * (Tr::dtype == BFJ_FP8 ? 4 : 1) >= BFJ_PIPE_MFMA;
+    // One candidate block cb against this wavefront's rows.  With `more`, next(c)
+    // gives the next block's K-chunk c, which is read into B[c] as soon as this
+    // block's MFMAs on B[c] have issued: the next block's LDS reads are in flight
+    // under this block's remaining MFMAs instead of stalling its first one, with no
+    // extra registers.
+    auto block = [&](int cb, frag_t (&B)[KC], val_t nj, int SB, bool more, auto &&next) {
         // acc = g_i + xhat_i . xhat_j; the first MFMA reads g_i as its C operand, so no
         // VALU work precedes the MFMAs.  Z = acc - n_j is only formed for hot tiles:
         // acc < n_j is Z < 0 exactly, and max commutes with subtracting the per-lane n_j.
@@ -296,10 +309,21 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][0], B[0], G[rb], SA[rb], SB);
+        // The scheduling barriers keep each read right after the MFMAs it follows
+        // (left alone, the scheduler sinks them all to the block's end).
+        if (PIPE && more) {
+            B[0] = next(0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
 #pragma unroll
-        for (int c = 1; c < KC; ++c)
+        for (int c = 1; c < KC; ++c) {
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) acc[rb] = Tr::mma(A[rb][c], B[c], acc[rb], SA[rb], SB);
+            if (PIPE && more) {
+                B[c] = next(c);
+                if (c + 1 < KC) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
 
         // Inputs are NaN-free (the pass rejects non-finite rows), so with -fno-honor-nans
         // these are v_max3 trees: one per row block, then one over the blocks.
@@ -353,6 +377,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             if (Tr::dtype == BFJ_FP8) SBn = (int)sc[cb * 16 + (lane & 15)];
         };
         fetch(cb0);
+        auto none = [&](int) { return frag_t{}; };
         for (int cb = cb0; cb < cb1; ++cb) {
             frag_t B[KC];
 #pragma unroll
@@ -360,7 +385,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             const val_t nj = njn;
             const int SB = SBn;
             if (cb + 1 < cb1) fetch(cb + 1);
-            if (any_rows) block(cb, B, nj, SB);
+            if (any_rows) block(cb, B, nj, SB, false, none);
         }
         return;
     }
@@ -389,14 +414,34 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         issue(t + L::NB - 1, rcur == 0 ? L::NB - 1 : rcur - 1); // into the buffer of slot t-1
         if (!any_rows) return;
         const unsigned char *b = lds + rcur * L::BUF;
+        auto frag = [&](int h, int c) { return reinterpret_cast<const frag_t *>(b)[(h * KC + c) * BFJ_WAVE + lane]; };
+        auto norm = [&](int h) { return reinterpret_cast<const val_t *>(b + L::FRAG)[h * 16 + (lane & 15)]; };
+        auto scale = [&](int h) {
+            return (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + h * 16 + (lane & 15)] : 127;
+        };
+        frag_t B[KC];
+        if (PIPE) {
+            // The slot's first fragments and all its norms (and FP8 scales) up front;
+            // each block then reads the next one's fragments under its own MFMAs.
+            val_t njs[L::CPT];
+            int SBs[L::CPT];
 #pragma unroll
-        for (int h = 0; h < L::CPT; ++h) {
-            frag_t B[KC];
+            for (int c = 0; c < KC; ++c) B[c] = frag(0, c);
 #pragma unroll
-            for (int c = 0; c < KC; ++c) B[c] = reinterpret_cast<const frag_t *>(b)[(h * KC + c) * BFJ_WAVE + lane];
-            const val_t nj = reinterpret_cast<const val_t *>(b + L::FRAG)[h * 16 + (lane & 15)];
-            const int SB = (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + h * 16 + (lane & 15)] : 127;
-            block(cb0 + t * L::CPT + h, B, nj, SB);
+            for (int h = 0; h < L::CPT; ++h) {
+                njs[h] = norm(h);
+                SBs[h] = scale(h);
+            }
+#pragma unroll
+            for (int h = 0; h < L::CPT; ++h)
+                block(cb0 + t * L::CPT + h, B, njs[h], SBs[h], h + 1 < L::CPT, [&](int c) { return frag(h + 1, c); });
+        } else {
+#pragma unroll
+            for (int h = 0; h < L::CPT; ++h) {
+#pragma unroll
+                for (int c = 0; c < KC; ++c) B[c] = frag(h, c);
+                block(cb0 + t * L::CPT + h, B, norm(h), scale(h), false, [&](int c) { return frag(h, c); });
+            }
         }
     };
     // Unrolled by NB so that every ring address is a constant.
