@@ -95,7 +95,7 @@ struct WorkItem {
 // streams these into per-wavefront chunks; k_classify applies the per-eps tests.
 struct Candidate {
     uint32_t row; // global row (row_off + dataset-local row)
-    uint32_t col; // bits 0..30: dataset-local candidate row; bit 31: the pair counts for it too
+    uint32_t col; // bits 0..30: global candidate row (row_off + dataset-local row); bit 31: the pair counts for it too
     double z;     // Z of the pair (kernel space)
 };
 #define BFJ_COL_SYM 0x80000000u

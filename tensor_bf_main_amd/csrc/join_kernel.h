@@ -156,7 +156,7 @@ __device__ inline void cand_close(const JoinArgs &a, const CandCursor &cc, int l
 template <class Tr>
 __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &cc, const typename Tr::acc_t &v,
                                                 typename Tr::val_t nj, int col, bool col_ok, int li0, bool sym,
-                                                uint32_t row0, int lane)
+                                                uint32_t row0, uint32_t row_off, int lane)
 {
     unsigned long long bal[4];
     uint32_t cnt[4], nrec = 0;
@@ -185,7 +185,7 @@ __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &c
             if ((bal[r] >> lane) & 1ull) {
                 Candidate rec;
                 rec.row = row0 + (uint32_t)Tr::out_row(lane, r);
-                rec.col = (uint32_t)col | tag;
+                rec.col = (row_off + (uint32_t)col) | tag;
                 rec.z = (double)(v[r] - nj);
                 dst[off + (uint32_t)__popcll(bal[r] & below)] = rec;
             }
@@ -359,7 +359,8 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 #pragma unroll
                 for (int k = 1; k < RB; ++k)
                     if (rb == k) v = acc[k];
-                cand_push_block<Tr>(a, cc, v, nj, col, col_ok, blk * 16, sym, (uint32_t)(s.row_off + blk * 16), lane);
+                cand_push_block<Tr>(a, cc, v, nj, col, col_ok, blk * 16, sym, (uint32_t)(s.row_off + blk * 16),
+                                    (uint32_t)s.row_off, lane);
             }
         }
     };

@@ -29,7 +29,8 @@
 // global row and bucket are summed first (a vote loop over the distinct keys of the
 // wavefront).  `li` is the dataset-local row of `key` in dataset `s`.  All lanes take
 // part.
-__device__ inline void add_bucket(const PassArgs &a, uint32_t key, const SetDesc &s, uint32_t li, int bucket, int lane)
+template <class Set>
+__device__ inline void add_bucket(const PassArgs &a, uint32_t key, const Set &s, uint32_t li, int bucket, int lane)
 {
     const unsigned long long k = bucket >= 0 ? (((unsigned long long)key << 3) | (unsigned)bucket) : ~0ull;
     unsigned long long pending = __ballot(bucket >= 0);
@@ -113,7 +114,8 @@ __device__ inline void ctab_flush(const PassArgs &a, CountTable &t, int lane)
 // cnt[bi][i] += 1 and cnt[bj][j] += 1 for rows i, j of dataset s (bucket -1: no
 // count), through the table when use_tab holds (uniform), else by the vote loop.
 // All lanes take part.
-__device__ inline void count_rows(const PassArgs &a, CountTable &tab, bool use_tab, const SetDesc &s, uint32_t i,
+template <class Set>
+__device__ inline void count_rows(const PassArgs &a, CountTable &tab, bool use_tab, const Set &s, uint32_t i,
                                   int bi, uint32_t j, int bj, int lane)
 {
     if (use_tab) {
@@ -160,10 +162,13 @@ __device__ inline void append_pairs(const PassArgs &a, const SetDesc &s, bool ta
 // distance S (kernel space) and its quantisation radius r = q_i + q_j:
 //   certainly within eps if hi <= L = (slo - r)^2, certainly outside if lo > H = (shi + r)^2
 // (rounded outwards), in the band otherwise.
-__device__ inline void decide(const SetScalars &sc, uint32_t band_mask, int n_eps, double r, double lo, double hi,
+template <class Set>
+__device__ inline void decide(const Set &sc, uint32_t band_mask, int n_eps, double r, double lo, double hi,
                               uint32_t &in, uint32_t &band)
 {
-    for (int e = 0; e < n_eps; ++e) {
+#pragma unroll
+    for (int e = 0; e < BFJ_MAXEPS; ++e) {
+        if (e >= n_eps) break;
         if (!((band_mask >> e) & 1u)) continue;
         const double slo = sc.slo[e], shi = sc.shi[e];
         const double L = slo > r ? (slo - r) * (slo - r) * (1.0 - 0x1p-49) : -INFINITY;
@@ -183,31 +188,81 @@ struct Decided {
     uint32_t i, j, in_i, band_i, in_j, band_j;
 };
 
-__device__ inline Decided decide_record(const PassArgs &a, const Candidate *rec, bool live)
+// What classify needs of a dataset: its descriptor's offsets and its per-eps bounds.
+struct SetView {
+    int64_t row_off, cnt_off, npad;
+    int n_eps;
+    uint32_t band_mask;
+    double slo[BFJ_MAXEPS], shi[BFJ_MAXEPS];
+};
+
+// Through the constant address space: scalar loads when si is uniform.  Both arrays
+// are read-only while classify runs.
+typedef const __attribute__((address_space(4))) SetDesc *bfj_cset;
+typedef const __attribute__((address_space(4))) SetScalars *bfj_cscal;
+
+__device__ inline SetView set_view_uniform(const PassArgs &a, int si)
+{
+    SetView v;
+    const bfj_cset s = (bfj_cset)a.sets + si;
+    const bfj_cscal sc = (bfj_cscal)a.scal + si;
+    v.row_off = s->row_off;
+    v.cnt_off = s->cnt_off;
+    v.npad = s->npad;
+    v.n_eps = s->n_eps;
+    v.band_mask = (uint32_t)sc->band_mask;
+#pragma unroll
+    for (int e = 0; e < BFJ_MAXEPS; ++e) {
+        v.slo[e] = sc->slo[e];
+        v.shi[e] = sc->shi[e];
+    }
+    return v;
+}
+
+// The loads of one candidate record: the record and the per-row data of both its rows
+// (the record holds both as global rows, so nothing here waits on the dataset's
+// descriptor).  Branch-free, so that the loads of several groups go out together.
+// Lanes past the chunk's fill re-read its last record and decide nothing.
+struct Fetched {
+    Candidate c;
+    double2 h;     // he[row]
+    double qi, qj; // quantisation radii of both rows
+    int si;        // dataset
+    bool live;
+};
+
+__device__ inline void fetch_record(const PassArgs &a, const Candidate *recs, uint32_t t, uint32_t fill, Fetched &f)
+{
+    f.live = t < fill;
+    f.c = recs[f.live ? t : fill - 1];
+    f.si = (int)a.row_set[f.c.row];
+    f.h = a.he[f.c.row];
+    f.qi = a.qerr[f.c.row];
+    f.qj = a.qerr[f.c.col & ~BFJ_COL_SYM];
+}
+
+// s: the record's dataset (row_off); b: its bounds (slo, shi); n_eps, band_mask of it.
+template <class Set, class Bounds>
+__device__ inline Decided decide_fetched(const Fetched &f, const Set &s, const Bounds &b, int n_eps,
+                                         uint32_t band_mask)
 {
     Decided o;
-    o.c.row = 0;
-    o.c.col = 0;
-    o.c.z = -1.0;
-    if (live) o.c = *rec;
-    o.si = live ? (int)a.row_set[o.c.row] : 0;
-    const SetDesc &s = a.sets[o.si];
-    o.i = o.c.row - (uint32_t)s.row_off;
-    o.j = o.c.col & ~BFJ_COL_SYM;
+    o.c = f.c;
+    o.si = f.si;
+    o.i = f.c.row - (uint32_t)s.row_off;
+    o.j = (f.c.col & ~BFJ_COL_SYM) - (uint32_t)s.row_off;
     o.in_i = o.band_i = o.in_j = o.band_j = 0;
-    if (live) {
+    if (f.live) {
         // S_hat = 2 g_i + N_i - 2 Z, |S_hat - S| <= E_i; mg covers the FP64 rounding
         // of these few operations.
-        const double2 h = a.he[o.c.row];
-        const double shat = h.x - 2.0 * o.c.z;
-        const double mg = (fabs(h.x) + 2.0 * fabs(o.c.z) + h.y) * 0x1p-50;
-        const double lo = shat - h.y - mg, hi = shat + h.y + mg;
+        const double shat = f.h.x - 2.0 * f.c.z;
+        const double mg = (fabs(f.h.x) + 2.0 * fabs(f.c.z) + f.h.y) * 0x1p-50;
+        const double lo = shat - f.h.y - mg, hi = shat + f.h.y + mg;
         // |dist(xhat_i, xhat_j) - dist(y_i, y_j)| <= q_i + q_j: the same bounds hold
         // for both rows of the pair.
-        const double r = (a.qerr[o.c.row] + a.qerr[s.row_off + o.j]) * (1.0 + 0x1p-50);
-        const SetScalars &sc = a.scal[o.si];
-        decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, lo, hi, o.in_i, o.band_i);
-        if (o.c.col & BFJ_COL_SYM) {
+        const double r = (f.qi + f.qj) * (1.0 + 0x1p-50);
+        decide(b, band_mask, n_eps, r, lo, hi, o.in_i, o.band_i);
+        if (f.c.col & BFJ_COL_SYM) {
             o.in_j = o.in_i;
             o.band_j = o.band_i;
         }
@@ -215,8 +270,46 @@ __device__ inline Decided decide_record(const PassArgs &a, const Candidate *rec,
     return o;
 }
 
+#ifndef BFJ_CLS_GROUPS
+#define BFJ_CLS_GROUPS 4 // 64-record groups of a chunk whose loads are in flight together
+#endif
+
+// body(decided, dataset, group) for the groups g0 + k, k < BFJ_CLS_GROUPS, with bit k
+// of gmask (uniform).  When every record of these groups is from one dataset (the
+// common case: a chunk is filled by one join wavefront), its descriptor and scalars
+// are read once through scalar loads; otherwise per lane.  All lanes take part.
+template <class Body>
+__device__ inline void classify_groups(const PassArgs &a, const Candidate *recs, uint32_t fill, uint32_t g0,
+                                       uint32_t gmask, int lane, Body &&body)
+{
+    constexpr int G = BFJ_CLS_GROUPS;
+    Fetched f[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) fetch_record(a, recs, (g0 + k) * BFJ_WAVE + lane, fill, f[k]);
+    const int si0 = __builtin_amdgcn_readfirstlane(f[0].si);
+    bool same = true;
+#pragma unroll
+    for (int k = 0; k < G; ++k) same &= f[k].si == si0;
+    if (__ballot(!same) == 0ull) {
+        const SetView v = set_view_uniform(a, si0);
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if ((gmask >> k) & 1u) body(decide_fetched(f[k], v, v, v.n_eps, v.band_mask), v, g0 + k);
+    } else {
+        // Per lane, the bounds read as the eps loop needs them.
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if ((gmask >> k) & 1u) {
+                const SetDesc &s = a.sets[f[k].si];
+                const SetScalars &sc = a.scal[f[k].si];
+                body(decide_fetched(f[k], s, sc, s.n_eps, (uint32_t)sc.band_mask), s, g0 + k);
+            }
+    }
+}
+
 __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
 {
+    constexpr int G = BFJ_CLS_GROUPS;
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
     __shared__ CountTable tabs[BFJ_BLOCK / BFJ_WAVE];
@@ -229,22 +322,25 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
     for (uint32_t ch = wave_id; ch < n_chunks; ch += n_waves) {
         const uint32_t fill = min(a.chunk_fill[ch], (uint32_t)BFJ_CHUNK);
         const Candidate *recs = a.cand + (size_t)ch * BFJ_CHUNK;
+        const uint32_t n_groups = (fill + BFJ_WAVE - 1) / BFJ_WAVE; // at most 16
         // Pass 1: count the certain-in pairs, note which records go to the refine list.
         uint32_t band_groups = 0, n_band = 0;
-        for (uint32_t t0 = 0, g = 0; t0 < fill; t0 += BFJ_WAVE, ++g) {
-            const bool live = t0 + lane < fill;
-            const Decided d = decide_record(a, recs + t0 + lane, live);
-            const SetDesc &s = a.sets[d.si];
+        auto count_body = [&](const Decided &d, const auto &s, uint32_t g) {
             // A record with an eps in the band is decided whole by the FP64 refine;
             // otherwise it counts once, under the first eps that certainly holds it.
             const bool band = (d.band_i | d.band_j) != 0u;
             const int bi = (!band && d.in_i) ? __builtin_ctz(d.in_i) : -1;
             const int bj = (!band && d.in_j) ? __builtin_ctz(d.in_j) : -1;
             count_rows(a, tab, use_tab, s, d.i, bi, d.j, bj, lane);
-            if (a.emit_pairs) append_pairs(a, s, !band && (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane);
+            if (a.emit_pairs) append_pairs(a, a.sets[d.si], !band && (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane);
             const uint32_t nb = (uint32_t)__popcll(__ballot(band));
             if (nb) band_groups |= 1u << g;
             n_band += nb;
+        };
+        for (uint32_t g0 = 0; g0 < n_groups; g0 += G) {
+            const uint32_t left = n_groups - g0;
+            classify_groups(a, recs, fill, g0, left >= (uint32_t)G ? (1u << G) - 1u : (1u << left) - 1u, lane,
+                            count_body);
         }
         if (!n_band) continue;
         // One atomic per chunk reserves its refine-list records (a counter that every
@@ -253,10 +349,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
         if (lane == 0) base = atomicAdd(a.band_count, n_band);
         base = __shfl(base, 0);
         // Pass 2: decide the band groups again and write their records.
-        for (uint32_t t0 = 0, g = 0; t0 < fill; t0 += BFJ_WAVE, ++g) {
-            if (!((band_groups >> g) & 1u)) continue;
-            const bool live = t0 + lane < fill;
-            const Decided d = decide_record(a, recs + t0 + lane, live);
+        auto band_body = [&](const Decided &d, const auto &, uint32_t) {
             const bool band = (d.band_i | d.band_j) != 0u;
             const unsigned long long bb = __ballot(band);
             if (band) {
@@ -271,6 +364,10 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
                 }
             }
             base += (uint32_t)__popcll(bb);
+        };
+        for (uint32_t g0 = 0; g0 < n_groups; g0 += G) {
+            const uint32_t m = (band_groups >> g0) & ((1u << G) - 1u);
+            if (m) classify_groups(a, recs, fill, g0, m, lane, band_body);
         }
     }
     if (use_tab) ctab_flush(a, tab, lane);
