@@ -449,7 +449,7 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
     RC(ensure(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
     RC(ensure(ctx->counters, 64 * 4));
-    RC(ensure(ctx->colsum, std::max<size_t>(p.blk_set.size(), 1) * BFJ_MAXDIM * 8));
+    RC(ensure(ctx->colsum, std::max<size_t>(p.blk_set.size(), 1) * BFJ_MAXDIM * 8 * 3)); // sums, maxima, minima
     RC(ensure(ctx->mu, p.sets.size() * BFJ_MAXDIM * 8));
     RC(ensure(ctx->sets, p.sets.size() * sizeof(SetDesc)));
     RC(ensure(ctx->scal, p.sets.size() * sizeof(SetScalars)));

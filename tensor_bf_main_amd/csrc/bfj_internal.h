@@ -149,7 +149,7 @@ struct PassArgs {
     int32_t n_row_blocks;      // blocks in those tables (64 rows each)
     const int32_t *set_blk0;   // set -> its first block in the tables
     const int32_t *set_nblk;   // set -> number of its blocks
-    double *colsum;            // [row_blocks][BFJ_MAXDIM] partial column sums
+    double *colsum;            // [3][row_blocks][BFJ_MAXDIM] per-block column sums, maxima, minima
     double *mu;                // [n_sets][BFJ_MAXDIM]
     unsigned char *pack;
     void *nq, *gq;

@@ -75,69 +75,78 @@ __device__ inline double wave_max(double v)
 // ---- column means -----------------------------------------------------------
 // Partial sums over 64-row blocks, then an in-order sum per dataset: the mean is
 // the same bits on every run.
+// The same pass keeps each block's column maxima and minima (colsum holds three
+// [row block][BFJ_MAXDIM] arrays: sums, maxima, minima) for the scale.
 __global__ void k_colsum(PassArgs a)
 {
     const int b = blockIdx.x;
     const SetDesc &s = a.sets[a.blk_set[b]];
     const int64_t r0 = a.blk_row0[b];
+    const int64_t plane = (int64_t)a.n_row_blocks * BFJ_MAXDIM;
     for (int k = threadIdx.x; k < s.d; k += blockDim.x) {
-        double acc = 0.0;
-        for (int64_t i = r0; i < r0 + BFJ_PREP_ROWS && i < s.n; ++i) acc += s.x[i * s.ld + k];
+        double acc = 0.0, mx = -INFINITY, mn = INFINITY;
+        for (int64_t i = r0; i < r0 + BFJ_PREP_ROWS && i < s.n; ++i) {
+            const double v = s.x[i * s.ld + k];
+            acc += v;
+            mx = fmax(mx, v);
+            mn = fmin(mn, v);
+        }
         a.colsum[(int64_t)b * BFJ_MAXDIM + k] = acc;
+        a.colsum[plane + (int64_t)b * BFJ_MAXDIM + k] = mx;
+        a.colsum[2 * plane + (int64_t)b * BFJ_MAXDIM + k] = mn;
     }
 }
 
 // One block per (dataset, column): the block sums the column's partial sums in a
 // fixed order (strided per thread, then a tree), so the mean is reproducible.
+// It also reduces the column's extremes to max |x - mu| over the column, which goes
+// into the dataset's max |x - mu|: fl(x - mu) is monotonic in x, so this is the
+// largest fl(|x - mu|) over the column's elements exactly.  NaN or inf in the
+// column makes the sum (and the mean) or an extreme non-finite; that marks the
+// dataset.
 __global__ void k_colmean(PassArgs a)
 {
-    __shared__ double red[BFJ_BLOCK];
+    __shared__ double red[BFJ_BLOCK], rmx[BFJ_BLOCK], rmn[BFJ_BLOCK];
     const int si = blockIdx.x, k = blockIdx.y;
     const SetDesc &s = a.sets[si];
-    double acc = 0.0;
+    const int64_t plane = (int64_t)a.n_row_blocks * BFJ_MAXDIM;
+    double acc = 0.0, mx = -INFINITY, mn = INFINITY;
     if (k < s.d) {
         const int64_t b0 = a.set_blk0[si];
-        for (int b = threadIdx.x; b < a.set_nblk[si]; b += BFJ_BLOCK) acc += a.colsum[(b0 + b) * BFJ_MAXDIM + k];
+        for (int b = threadIdx.x; b < a.set_nblk[si]; b += BFJ_BLOCK) {
+            const int64_t o = (b0 + b) * BFJ_MAXDIM + k;
+            acc += a.colsum[o];
+            mx = fmax(mx, a.colsum[plane + o]);
+            mn = fmin(mn, a.colsum[2 * plane + o]);
+        }
     }
     red[threadIdx.x] = acc;
+    rmx[threadIdx.x] = mx;
+    rmn[threadIdx.x] = mn;
     __syncthreads();
     for (int h = BFJ_BLOCK / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        if ((int)threadIdx.x < h) {
+            red[threadIdx.x] += red[threadIdx.x + h];
+            rmx[threadIdx.x] = fmax(rmx[threadIdx.x], rmx[threadIdx.x + h]);
+            rmn[threadIdx.x] = fmin(rmn[threadIdx.x], rmn[threadIdx.x + h]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.mu[(int64_t)si * BFJ_MAXDIM + k] = (s.n > 0 && k < s.d) ? red[0] / (double)s.n : 0.0;
+    if (threadIdx.x == 0) {
+        const bool col = s.n > 0 && k < s.d;
+        const double mu = col ? red[0] / (double)s.n : 0.0;
+        a.mu[(int64_t)si * BFJ_MAXDIM + k] = mu;
+        if (col) {
+            const double m = fmax(rmx[0] - mu, mu - rmn[0]);
+            if (!isfinite(mu) || !isfinite(rmx[0]) || !isfinite(rmn[0]) || !isfinite(m))
+                atomicOr(&a.scal[si].nonfinite, 1ull);
+            else
+                atomic_max_nonneg(&a.scal[si].maxabs_bits, m);
+        }
+    }
 }
 
 // ---- scale ------------------------------------------------------------------
-__global__ void k_maxabs(PassArgs a)
-{
-    __shared__ double red[BFJ_BLOCK / BFJ_WAVE];
-    const int b = blockIdx.x;
-    const int si = a.blk_set[b];
-    const SetDesc &s = a.sets[si];
-    const double *mu = a.mu + (int64_t)si * BFJ_MAXDIM;
-    const int64_t r0 = a.blk_row0[b];
-    double m = 0.0;
-    bool bad = false;
-    for (int64_t t = threadIdx.x; t < (int64_t)BFJ_PREP_ROWS * s.d; t += blockDim.x) {
-        const int64_t i = r0 + t / s.d;
-        const int k = (int)(t % s.d);
-        if (i < s.n) {
-            const double v = fabs(s.x[i * s.ld + k] - mu[k]);
-            bad |= !isfinite(v);
-            m = fmax(m, v);
-        }
-    }
-    if (bad) atomicOr(&a.scal[si].nonfinite, 1ull);
-    m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < BFJ_BLOCK / BFJ_WAVE; ++w) m = fmax(m, red[w]);
-        atomic_max_nonneg(&a.scal[si].maxabs_bits, red[0] > m ? red[0] : m);
-    }
-}
-
 // sigma = 2^ceil(log2(max|x - mu|)) so the centred, scaled rows lie in [-1, 1]
 // for every dtype: the FP16/FP8 ranges are respected and no norm can overflow.
 __global__ void k_sigma(PassArgs a)
@@ -536,7 +545,6 @@ void bfj_launch_prep(const PassArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(k_colsum, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_colmean, dim3(a.n_sets, BFJ_MAXDIM), dim3(BFJ_BLOCK), 0, st, a);
-    hipLaunchKernelGGL(k_maxabs, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_sigma, dim3((a.n_sets + 255) / 256), dim3(256), 0, st, a);
     bfj_launch_order(a, st);
     if (a.dtype == BFJ_FP16) hipLaunchKernelGGL(k_quantize16<false>, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
