@@ -252,7 +252,7 @@ def main():
                     help="only run one device-to-device copy of MIB MiB (scales HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the join runs on a single MI355X in this build (asked for {args.gpus}).",
+        print(f"bench.py measures one GPU: the join runs on a single MI355X (asked for {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
 

@@ -52,7 +52,7 @@
 #endif
 
 #ifndef ADDITIONAL_POINTS
-#define ADDITIONAL_POINTS 15 /* params.h:20: zero rows appended by the driver */
+#define ADDITIONAL_POINTS 15 /* params.h:20: zero rows main.cu appends after the data */
 #endif
 
 /* params.h:160-164: positional CLI arguments */

@@ -41,7 +41,7 @@ def test_golden_reference_files():
         counts, totals = oracle.join(rows, g["eps"])
         assert [int(t) for t in totals] == g["totals"]
         np.testing.assert_array_equal(counts, np.asarray(g["counts"]))
-    # The survey's numpy values for the reference's files.
+    # Independent numpy FP64 totals for the reference's two sample files.
     assert gold["test100.txt"]["totals"][gold["test100.txt"]["eps"].index(0.05)] == 544
     assert gold["test128.txt"]["totals"][gold["test128.txt"]["eps"].index(0.05)] == 828
 
