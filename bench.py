@@ -241,6 +241,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=3, choices=[2, 3, 4, 5])
     ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--d", type=int, default=0, help="dimension of workload 2 (uniform; 0 = 128)")
     ap.add_argument("--sets", type=int, default=512)
     ap.add_argument("--dtype", default="", choices=["", "fp16", "bf16", "fp8", "fp64"])
     ap.add_argument("--fp64-steps", type=int, default=2)
@@ -273,8 +274,12 @@ def main():
         if args.tile_rows:
             ctx.set_tile_rows(dtype, args.tile_rows)
         log(f"generating workload {args.workload}: n={n}")
-        x, eps = wl.config(args.workload, n=n)
-        names = {2: ("pair-distances/s at n=100k,d=128 (FP64-MFMA + FP64 band refine, exact counts)", "uniform"),
+        if args.d and args.workload != 2:
+            sys.exit("--d applies to workload 2 only")
+        x, eps = wl.config(args.workload, n=n, d=args.d or None)
+        w2 = ("pair-distances/s at n=100k,d=128 (FP64-MFMA + FP64 band refine, exact counts)" if not args.d else
+              f"pair-distances/s at n={n},d={args.d} ({dtype.upper()}-MFMA + FP64 band refine, exact counts)")
+        names = {2: (w2, "uniform" if not args.d else f"uniform_d{args.d}"),
                  3: ("pair-distances/s at n=1M,d=128 (FP16-MFMA + FP64 band refine, exact counts)",
                      "gmm_64c_sigma0.03"),
                  4: (f"pair-distances/s at n=500k,d=256 ({dtype.upper()}-MFMA + FP64 band refine, exact counts)",

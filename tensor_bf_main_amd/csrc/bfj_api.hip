@@ -133,18 +133,20 @@ static int lds_for(int dtype, int kc, int rows, int n_eps)
     return 0;
 }
 
-// Default tile: the most resident blocks per CU, counted up to a cap (latency hiding
-// across blocks matters more than strip width up to there), then the widest strip.
-// Measured on MI355X: FP16/BF16/FP64 peak at 3 blocks per CU with wider strips, FP8
-// (twice the MFMA rate, so more issue and LDS work per flop) at 4.
-// Tile rows R: the widest strip among those with the most resident blocks per CU,
-// counted up to a cap.  Measured: FP8 wants 4 blocks per CU and FP16/BF16 3 in input
-// order (R = 64 at d = 128); in the locality order hot tiles are rare, and 2 blocks of
-// wider strips win (FP16 d = 128: R = 128, 76 ms against 78 ms at R = 64).
+// Default tile rows R: the widest strip among those with the most resident blocks per
+// CU, counted up to a cap (latency hiding across blocks matters more than strip width
+// up to there).  Measured on MI355X:
+//  * FP8 (twice the MFMA rate, so more issue and LDS work per flop) wants 4 blocks per
+//    CU, FP16/BF16/FP64 3 in input order (R = 64 at d = 128);
+//  * in the locality order hot tiles are rare and 2 blocks of wider strips win
+//    (FP16 d = 128: R = 128, 76 ms against 78 ms at R = 64);
+//  * FP16/BF16 with 1-2 K-chunks (d <= 64): a candidate block is too little MFMA work
+//    to cover its epilogue, so resident wavefronts matter more than strip width
+//    (n = 1M: R = 64 beats R = 128 by 8 % at d = 16 and d = 64).
 static int heuristic_rows(int dtype, int kc, int n_eps, bool reordered)
 {
     const bool low = dtype == BFJ_FP16 || dtype == BFJ_BF16;
-    const int cap = dtype == BFJ_FP8 ? 4 : (low && reordered ? 2 : 3);
+    const int cap = dtype == BFJ_FP8 ? 4 : (low && kc <= 2 ? 4 : (low && reordered ? 2 : 3));
     int best = 0, best_occ = -1;
     for (int r : kRowOptions)
         if (bfj_join_has(dtype, kc, r)) {

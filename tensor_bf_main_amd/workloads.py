@@ -68,8 +68,9 @@ def eps_quantile(x: np.ndarray, q, sample: int, seed: int):
     return picks if np.ndim(q) else picks[0]
 
 
-def config(k: int, n: int | None = None):
-    """(data, eps) of workload k (1..4) with optional reduced n (same recipe, same seed)."""
+def config(k: int, n: int | None = None, d: int | None = None):
+    """(data, eps) of workload k (1..4) with optional reduced n (same recipe, same seed);
+    d overrides the dimension of the uniform workload 2."""
     seed = SEED_BASE + k
     if k == 1:
         n = n or 10_000
@@ -77,7 +78,7 @@ def config(k: int, n: int | None = None):
         return x, eps_quantile(x, 0.005, 1000, seed + 100)
     if k == 2:
         n = n or 100_000
-        x = uniform(n, 128, seed)
+        x = uniform(n, d or 128, seed)
         return x, eps_quantile(x, 64.0 / (n - 1), 2000, seed + 100)
     if k == 3:
         n = n or 1_000_000
