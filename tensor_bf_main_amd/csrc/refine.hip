@@ -499,7 +499,9 @@ __device__ inline void refine_flush(const PassArgs &a, int err_set, double err_r
 // Each wavefront takes runs of BFJ_REFINE_RUN consecutive records (classify writes a
 // chunk's records together, so the rows of a run repeat and their counts meet in
 // the wavefront's count table), the runs dealt round-robin over the wavefronts.
+#ifndef BFJ_REFINE_RUN
 #define BFJ_REFINE_RUN 512
+#endif
 __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
