@@ -771,6 +771,13 @@ static int join_one(bfj_ctx *ctx, const double *d_x, int64_t n, int32_t d, int64
     h.n_eps = n_eps;
     h.out = d_counts;
     for (int e = 0; e < n_eps && e < BFJ_MAXEPS; ++e) h.eps[e] = eps[e];
+    if (n == 0) { // an empty dataset: nothing to launch, every total is 0
+        RC(validate(h, dtype));
+        if (totals)
+            for (int e = 0; e < n_eps; ++e) totals[e] = 0;
+        if (stats) memset(stats, 0, sizeof *stats);
+        return BFJ_OK;
+    }
     PassPlan p;
     RC(build_plan(ctx, {h}, dtype, p, force_rows));
     RC(stage_plan(ctx, p));
@@ -839,6 +846,10 @@ int bfj_self_join_pairs(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, 
     ctx->emit_pairs = 0;
     ctx->pair_capacity = 0;
     RC(rc);
+    if (n == 0) {
+        if (total) *total = 0;
+        return BFJ_OK;
+    }
     unsigned long long found = 0;
     SetScalars sc;
     HC(hipMemcpy(&found, (uint32_t *)ctx->counters.p + 4, 8, hipMemcpyDeviceToHost));

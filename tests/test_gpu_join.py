@@ -4,6 +4,7 @@ The bar is bit-exact per-point counts and totals for every dtype: the
 low-precision paths decide only the pairs whose rigorous error band excludes
 the threshold and re-evaluate the rest in FP64 on the GPU.
 """
+import ctypes
 import json
 import os
 
@@ -317,3 +318,38 @@ def test_locality_order_gives_the_same_results(ctx, dtype, n, d, kind):
     assert np.all(pairs[:, 0] < pairs[:, 1])
     key = lambda p: np.sort(p[:, 0].astype(np.int64) * n + p[:, 1])
     np.testing.assert_array_equal(key(pairs), key(plain_pairs))
+
+
+def test_argument_errors_on_a_live_context(ctx):
+    """Bad arguments fail with BFJ_ERR_INVALID / BFJ_ERR_UNSUPPORTED and a message that
+    names them, before any device work, and leave the context usable; n = 0 is a valid
+    (empty) join."""
+    lib = ctx.lib
+    x = np.random.default_rng(5).random((64, 300))
+    eps1 = (ctypes.c_double * 1)(0.5)
+    tot = (ctypes.c_uint64 * 9)()
+
+    def call(n, d, ld, eps, n_eps, dtype=1, rb=0, re=None, data=True):
+        return lib.bfj_self_join(ctx._h, x.ctypes.data if data else None, n, d, ld, eps, n_eps, dtype, rb,
+                                 n if re is None else re, None, tot, None)
+
+    cases = [
+        (dict(n=64, d=0, ld=300, eps=eps1, n_eps=1), 1, b"d <= ld"),
+        (dict(n=64, d=257, ld=300, eps=eps1, n_eps=1), 4, b"257"),
+        (dict(n=64, d=20, ld=10, eps=eps1, n_eps=1), 1, b"d <= ld"),
+        (dict(n=-1, d=20, ld=300, eps=eps1, n_eps=1), 1, b"n < 0"),
+        (dict(n=64, d=20, ld=300, eps=(ctypes.c_double * 9)(*[0.1] * 9), n_eps=9), 1, b"n_eps"),
+        (dict(n=64, d=20, ld=300, eps=(ctypes.c_double * 1)(float("nan")), n_eps=1), 1, b"NaN"),
+        (dict(n=64, d=20, ld=300, eps=eps1, n_eps=1, rb=10, re=5), 1, b"row range"),
+        (dict(n=64, d=20, ld=300, eps=eps1, n_eps=1, re=65), 1, b"row range"),
+        (dict(n=64, d=20, ld=300, eps=eps1, n_eps=1, dtype=9), 1, b"dtype"),
+        (dict(n=64, d=20, ld=300, eps=eps1, n_eps=1, data=False), 1, b"NULL"),
+    ]
+    for kw, code, word in cases:
+        assert call(**kw) == code, kw
+        assert word in lib.bfj_last_error(), (kw, lib.bfj_last_error())
+    # An empty dataset joins to nothing.
+    assert call(n=0, d=20, ld=300, eps=eps1, n_eps=1, data=False) == 0
+    assert tot[0] == 0
+    # The context still works after the errors.
+    _check(ctx, x[:, :20], 0.5, "fp16")
