@@ -23,12 +23,14 @@
 // ---- classification of the join's candidate stream --------------------------------
 // One wavefront per chunk.  Per record, row and eps: certainly within eps is
 // counted, inside the band goes to the refine list, otherwise out.  A record of a
-// symmetric sweep decides for its row and for its candidate row.  Counts are summed
-// per distinct row of the wavefront before one atomic per row and eps.
-// cnt[bucket][row] += 1 for every lane with a bucket (>= 0); lanes with the same
-// global row and bucket are summed first (a vote loop over the distinct keys of the
-// wavefront).  `li` is the dataset-local row of `key` in dataset `s`.  All lanes take
-// part.
+// symmetric sweep decides for its row and for its candidate row.  Counts go through
+// a per-wavefront LDS table (below); add_bucket is the path for passes whose count
+// array is not indexable in 32 bits.
+//
+// add_bucket: cnt[bucket][row] += 1 for every lane with a bucket (>= 0); lanes with the
+// same global row and bucket are summed first (a vote loop over the distinct keys of
+// the wavefront), then one atomic per key.  `li` is the dataset-local row of `key` in
+// dataset `s`.  All lanes take part.
 template <class Set>
 __device__ inline void add_bucket(const PassArgs &a, uint32_t key, const Set &s, uint32_t li, int bucket, int lane)
 {
