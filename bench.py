@@ -72,9 +72,10 @@ def cpu_baseline(x, eps, rows_n, got_counts, seed=4242):
     oc = oracle.join_rows(x, rows, eps)
     tc = time.perf_counter() - t0
     mism = int((got_counts[:, rows] != oc).sum())
+    what = f"all {n} rows (full run)" if rows_n == n else f"{rows_n} seeded query rows"
     return {"value": rows_n * float(n) / tc, "unit": "pair-distances/s", "cores": oracle.num_threads(),
-            "kind": "port", "sample": f"{rows_n} seeded query rows x all {n} candidates, FP64 direct form "
-                                      f"(OpenMP), {tc:.1f} s"}, mism
+            "kind": "port", "sample": f"{what} x all {n} candidates, FP64 direct form "
+                                      f"(OpenMP), {tc:.3g} s"}, mism
 
 
 def single_dataset(args, ctx, x, eps, dtype, metric, workload):
@@ -150,7 +151,8 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
         out["fp64_join_tflops"] = flops / (r64.stats["ms_join"] * 1e-3) / 1e12
         log(f"fp64: {t64 * 1e3:.1f} ms/step; speedup {t64 / t_main:.2f}x; mismatches {mismatches}")
     if not args.skip_cpu:
-        rows_n = args.cpu_rows or max(64, min(12000, int(6.0e9 / n * 128 / d)))
+        # About 10-30 s of CPU work; the whole dataset when that is less (n = 10k).
+        rows_n = min(n, args.cpu_rows or max(64, min(12000, int(6.0e9 / n * 128 / d))))
         out["cpu_baseline"], sm = cpu_baseline(x, e, rows_n, got)
         out["oracle_sample_mismatches"] = sm
         mismatches = (mismatches or 0) + sm
@@ -239,7 +241,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", type=int, default=3, choices=[2, 3, 4, 5])
+    ap.add_argument("--workload", type=int, default=3, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--d", type=int, default=0, help="dimension of workload 2 (uniform; 0 = 128)")
     ap.add_argument("--sets", type=int, default=512)
@@ -269,8 +271,8 @@ def main():
         args.dtype = args.dtype or "fp16"
         out = batch_workload(args, ctx)
     else:
-        n = args.n or {2: 100_000, 3: 1_000_000, 4: 500_000}[args.workload]
-        dtype = args.dtype or {2: "fp64", 3: "fp16", 4: "bf16"}[args.workload]
+        n = args.n or {1: 10_000, 2: 100_000, 3: 1_000_000, 4: 500_000}[args.workload]
+        dtype = args.dtype or {1: "fp64", 2: "fp64", 3: "fp16", 4: "bf16"}[args.workload]
         if args.tile_rows:
             ctx.set_tile_rows(dtype, args.tile_rows)
         log(f"generating workload {args.workload}: n={n}")
@@ -279,7 +281,8 @@ def main():
         x, eps = wl.config(args.workload, n=n, d=args.d or None)
         w2 = ("pair-distances/s at n=100k,d=128 (FP64-MFMA + FP64 band refine, exact counts)" if not args.d else
               f"pair-distances/s at n={n},d={args.d} ({dtype.upper()}-MFMA + FP64 band refine, exact counts)")
-        names = {2: (w2, "uniform" if not args.d else f"uniform_d{args.d}"),
+        names = {1: ("pair-distances/s at n=10k,d=32 (FP64-MFMA + FP64 band refine, exact counts)", "uniform_n10k_d32"),
+                 2: (w2, "uniform" if not args.d else f"uniform_d{args.d}"),
                  3: ("pair-distances/s at n=1M,d=128 (FP16-MFMA + FP64 band refine, exact counts)",
                      "gmm_64c_sigma0.03"),
                  4: (f"pair-distances/s at n=500k,d=256 ({dtype.upper()}-MFMA + FP64 band refine, exact counts)",
