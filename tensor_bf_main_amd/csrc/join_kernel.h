@@ -304,7 +304,9 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         // issue, so the SIMD's matrix pipe goes to the wavefronts that can feed it
         // rather than to one that is loading or pushing candidates.  Measured: -0.8 %
         // at FP16 R = 64 (3 resident blocks per CU), but +4 % at BF16 R = 32 and
-        // +10 % for FP64, so only the wide low-precision strips use it.
+        // +10 % for FP64, so only the wide low-precision strips use it.  At FP16
+        // R = 128 the two s_setprio also bound the scheduler's regions: without them
+        // the kernel needs 276 registers and spills into AGPRs (1 wave per SIMD).
         constexpr bool PRIO = Tr::dtype != BFJ_FP64 && R >= 64;
         if (PRIO) __builtin_amdgcn_s_setprio(1);
 #pragma unroll
