@@ -195,8 +195,10 @@ struct bfj_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
     DevBuf pack, nq, gq, he, row_set, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
-        outbuf, cand, chunk_fill, pairs, ref_x, ref_aux, perm, label, ohist, oseg, pivots, pivot_norm;
-    int reorder = 1; // pack whole-dataset passes in a locality order (order.hip)
+        outbuf, cand, chunk_fill, pairs, ref_x, ref_aux, perm, label, ohist, oseg, pivots, pivot_norm, mid_rows,
+        mid_q, mid_band;
+    int reorder = 1;   // pack whole-dataset passes in a locality order (order.hip)
+    int mid_stage = 1; // FP8 passes: FP16 middle stage before the FP64 refine (refine.hip)
     uint32_t band_capacity = 0;
     uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
     int emit_pairs = 0;
@@ -441,6 +443,11 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
         ctx->band_capacity = (uint32_t)want;
     }
     RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+    if (p.dtype == BFJ_FP8 && ctx->mid_stage) {
+        RC(ensure(ctx->mid_rows, nrows * BFJ_MAXDIM * 2));
+        RC(ensure(ctx->mid_q, nrows * 8));
+        RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+    }
     if (ctx->chunk_capacity == 0) {
         // Start at 96 candidates per row (1.5 GiB at a million rows; about 100 neighbours
         // per point fit) and at least 4 chunks per join wavefront of the GPU; a pass
@@ -514,6 +521,11 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.band = (BandEntry *)ctx->band.p;
     a.band_count = (uint32_t *)ctx->counters.p;
     a.band_capacity = ctx->band_capacity;
+    a.mid = (p.dtype == BFJ_FP8 && ctx->mid_stage) ? 1 : 0;
+    a.mid_rows = (uint16_t *)ctx->mid_rows.p;
+    a.mid_q = (double *)ctx->mid_q.p;
+    a.mid_band = (BandEntry *)ctx->mid_band.p;
+    a.mid_count = (uint32_t *)ctx->counters.p + 8;
     a.cand = (const Candidate *)ctx->cand.p;
     a.chunk_fill = (const uint32_t *)ctx->chunk_fill.p;
     a.chunk_count = (const uint32_t *)ctx->counters.p + 1;
@@ -579,7 +591,15 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
     }
     if (timed) HC(hipEventRecord(ctx->ev[2], st));
     bfj_launch_classify(a, st);
-    bfj_launch_refine(a, st);
+    if (a.mid) {
+        bfj_launch_mid(a, st);
+        PassArgs r = a;
+        r.band = a.mid_band;
+        r.band_count = a.mid_count;
+        bfj_launch_refine(r, st);
+    } else {
+        bfj_launch_refine(a, st);
+    }
     bfj_launch_finalize(a, st);
     HC(hipGetLastError());
     if (timed) HC(hipEventRecord(ctx->ev[3], st));
@@ -617,6 +637,7 @@ static int grow_on_overflow(bfj_ctx *ctx, const uint32_t ctr[2], bool *again)
     if (band_count > ctx->band_capacity) {
         ctx->band_capacity = (uint32_t)std::min<uint64_t>((uint64_t)band_count + band_count / 4 + 1024, 0xffffffffull);
         RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+        if (ctx->mid_band.p) RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
         *again = true;
     }
     return BFJ_OK;
@@ -747,7 +768,8 @@ int bfj_destroy(bfj_ctx *c)
     DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
                       &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
                       &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux,
-                      &c->perm, &c->label, &c->ohist, &c->oseg, &c->pivots, &c->pivot_norm};
+                      &c->perm, &c->label, &c->ohist, &c->oseg, &c->pivots, &c->pivot_norm, &c->mid_rows,
+                      &c->mid_q, &c->mid_band};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);

@@ -162,6 +162,12 @@ struct PassArgs {
     BandEntry *band;
     uint32_t *band_count;
     uint32_t band_capacity;
+    // FP8 passes: a middle stage between classify and the FP64 refine (refine.hip).
+    int32_t mid;                       // 1: run it
+    uint16_t *mid_rows;                // [row][BFJ_MAXDIM] FP16 copy of the kernel-space rows, by position
+    double *mid_q;                     // [row] radius of that copy
+    BandEntry *mid_band;               // records it leaves for the FP64 refine (capacity band_capacity)
+    uint32_t *mid_count;
     const Candidate *cand;
     const uint32_t *chunk_fill;
     const uint32_t *chunk_count;
@@ -196,6 +202,7 @@ void bfj_launch_zero(void *p, size_t bytes, hipStream_t s);
 void bfj_launch_prep(const PassArgs &a, hipStream_t s);
 void bfj_launch_classify(const PassArgs &a, hipStream_t s);
 void bfj_launch_refine(const PassArgs &a, hipStream_t s);
+void bfj_launch_mid(const PassArgs &a, hipStream_t s);
 void bfj_launch_finalize(const PassArgs &a, hipStream_t s);
 void bfj_launch_all_pairs(uint2 *pairs, int64_t n, hipStream_t s);
 void bfj_launch_order(const PassArgs &a, hipStream_t s);

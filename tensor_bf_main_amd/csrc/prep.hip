@@ -147,6 +147,59 @@ __global__ void k_colmean(PassArgs a)
 }
 
 // ---- scale ------------------------------------------------------------------
+// FP8 passes: an FP16 copy of every kernel-space row, row-major by position
+// (BFJ_MAXDIM halves per row), and its radius ||y_i - fp16(y_i)|| bounded as in
+// k_quantize16.  The middle stage (refine.hip) decides most of the FP8 band from it,
+// reading whole 32-column chunks, so the columns from d up to a multiple of 32 are
+// written as zeros.  L lanes per row, 8 columns per lane.
+__global__ __launch_bounds__(BFJ_BLOCK) void k_mid_pack(PassArgs a)
+{
+    const int b = blockIdx.x;
+    const int si = a.blk_set[b];
+    const SetDesc &s = a.sets[si];
+    const double *mu = a.mu + (int64_t)si * BFJ_MAXDIM;
+    const double inv_sigma = 1.0 / a.scal[si].sigma; // exact: sigma is a power of two
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int units = (s.d + 31) / 32 * 4; // 8-column units, whole 32-column chunks
+    int L = 1;
+    while (L < units) L <<= 1;
+    const int rpp = BFJ_WAVE / L;
+    const int m = lane & (L - 1), lr = lane / L;
+    const int k0 = 8 * m;
+    const int64_t wrow0 = a.blk_row0[b] + wave * (BFJ_PREP_ROWS / 4);
+    for (int rr = 0; rr < BFJ_PREP_ROWS / 4; rr += rpp) {
+        const int64_t i = wrow0 + rr + lr;
+        const bool in = rr + lr < BFJ_PREP_ROWS / 4 && i < s.npad && m < units;
+        const bool real = in && i < s.n;
+        const int64_t src = real ? (int64_t)bfj_row_at(a, s, i) : 0;
+        double sq_e = 0.0, sq_y = 0.0;
+        uint32_t w[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            uint16_t bits[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int k = k0 + 2 * h + u;
+                const double v = (real && k < s.d) ? (s.x[src * s.ld + k] - mu[k]) * inv_sigma : 0.0;
+                const _Float16 q = (_Float16)v;
+                const double de = v - (double)q;
+                sq_e = fma(de, de, sq_e);
+                sq_y = fma(v, v, sq_y);
+                bits[u] = __builtin_bit_cast(uint16_t, q);
+            }
+            w[h] = (uint32_t)bits[0] | ((uint32_t)bits[1] << 16);
+        }
+        if (in)
+            *reinterpret_cast<uint4 *>(a.mid_rows + (s.row_off + i) * BFJ_MAXDIM + k0) = make_uint4(w[0], w[1], w[2], w[3]);
+        for (int o = L >> 1; o > 0; o >>= 1) {
+            sq_e += __shfl_xor(sq_e, o);
+            sq_y += __shfl_xor(sq_y, o);
+        }
+        if (m == 0 && rr + lr < BFJ_PREP_ROWS / 4 && i < s.npad)
+            a.mid_q[s.row_off + i] = real ? (sqrt(sq_e) + 2.3e-16 * sqrt(sq_y)) * (1.0 + 1e-9) + 1e-300 : 0.0;
+    }
+}
+
 // sigma = 2^ceil(log2(max|x - mu|)) so the centred, scaled rows lie in [-1, 1]
 // for every dtype: the FP16/FP8 ranges are respected and no norm can overflow.
 __global__ void k_sigma(PassArgs a)
@@ -550,5 +603,6 @@ void bfj_launch_prep(const PassArgs &a, hipStream_t st)
     if (a.dtype == BFJ_FP16) hipLaunchKernelGGL(k_quantize16<false>, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     else if (a.dtype == BFJ_BF16) hipLaunchKernelGGL(k_quantize16<true>, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     else hipLaunchKernelGGL(k_quantize, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
+    if (a.mid) hipLaunchKernelGGL(k_mid_pack, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_thresholds, dim3(a.n_row_blocks), dim3(BFJ_PREP_ROWS), 0, st, a);
 }

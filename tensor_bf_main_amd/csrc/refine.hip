@@ -596,3 +596,108 @@ void bfj_launch_finalize(const PassArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(k_finalize, dim3(a.n_row_blocks), dim3(BFJ_PREP_ROWS), 0, st, a);
 }
+
+// ---- FP8 middle stage -----------------------------------------------------------------
+// e4m3 keeps 3 mantissa bits, so the FP8 join's radius q_i is ~3.6 % of a row's norm
+// and its band holds orders of magnitude more pairs than FP16's.  Before the FP64 refine,
+// each FP8 band record is decided again from FP16 copies of its two rows (k_mid_pack,
+// 512 bytes per row instead of 2 KiB of FP64 rows): S = ||x16_i - x16_j||^2 in FP64
+// (each difference of two FP16 values is exact in FP64; the sum's rounding is far
+// below the 2^-40 margin taken), and |dist(x16_i, x16_j) - dist(y_i, y_j)| <=
+// q16_i + q16_j bounds the exact distance as in k_classify, with a radius ~128x
+// smaller.  Certain pairs are counted here; the rest go to the FP64 refine unchanged.
+// The same runs of records per wavefront and count table as k_refine.
+// sum of (p_k - q_k)^2 over 4 units (32 FP16 values) of both rows, in FP64 (exact
+// differences).
+__device__ inline double mid_chunk(const uint4 (&p)[4], const uint4 (&q)[4], double S)
+{
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const bfj_h8 hp = __builtin_bit_cast(bfj_h8, p[u]);
+        const bfj_h8 hq = __builtin_bit_cast(bfj_h8, q[u]);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double dk = (double)hp[k] - (double)hq[k];
+            S = fma(dk, dk, S);
+        }
+    }
+    return S;
+}
+
+__global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t cnt = *a.band_count;
+    const uint64_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
+    const uint64_t wave = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
+    const uint64_t step = (uint64_t)gridDim.x * (BFJ_BLOCK / BFJ_WAVE) * BFJ_REFINE_RUN;
+    __shared__ CountTable tabs[BFJ_BLOCK / BFJ_WAVE];
+    CountTable &tab = tabs[threadIdx.x >> 6];
+    const bool use_tab = a.cnt_len < (int64_t)BFJ_CTAB_EMPTY;
+    if (use_tab) ctab_clear(tab, lane);
+    for (uint64_t run = wave * BFJ_REFINE_RUN; run < m; run += step) {
+        const uint64_t end = run + BFJ_REFINE_RUN < m ? run + BFJ_REFINE_RUN : m;
+        for (uint64_t base = run; base < end; base += BFJ_WAVE) {
+            const uint64_t t = base + lane;
+            const bool live = t < end;
+            BandEntry be;
+            be.i = be.j = 0;
+            be.z = 0.f;
+            be.meta = 0;
+            if (live) be = a.band[t];
+            const SetDesc &s = a.sets[be.meta >> 16];
+            uint32_t in = 0, band = 0;
+            if (live) {
+                const SetScalars &sc = a.scal[be.meta >> 16];
+                const uint4 *p = reinterpret_cast<const uint4 *>(a.mid_rows + (s.row_off + be.i) * BFJ_MAXDIM);
+                const uint4 *q = reinterpret_cast<const uint4 *>(a.mid_rows + (s.row_off + be.j) * BFJ_MAXDIM);
+                // Chunks of 4 units (32 columns) through two register buffers, the next
+                // chunk's loads in flight while the current one is summed; k_mid_pack
+                // wrote the columns past d up to a multiple of 32 as zeros.
+                double S = 0.0;
+                const int nc = (s.d + 31) / 32;
+                uint4 pa[4], qa[4], pb[4], qb[4];
+                auto load = [&](int c, uint4 (&pp)[4], uint4 (&qq)[4]) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        pp[u] = p[4 * c + u];
+                        qq[u] = q[4 * c + u];
+                    }
+                };
+                load(0, pa, qa);
+                for (int c = 0; c < nc; c += 2) {
+                    if (c + 1 < nc) load(c + 1, pb, qb);
+                    S = mid_chunk(pa, qa, S);
+                    if (c + 1 >= nc) break;
+                    if (c + 2 < nc) load(c + 2, pa, qa);
+                    S = mid_chunk(pb, qb, S);
+                }
+                const double r = (a.mid_q[s.row_off + be.i] + a.mid_q[s.row_off + be.j]) * (1.0 + 0x1p-50);
+                decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, S * (1.0 - 0x1p-40), S * (1.0 + 0x1p-40), in, band);
+            }
+            // Still in a band: decided whole by the FP64 refine.  Otherwise the record
+            // counts once, under the first eps that certainly holds it (if any).
+            const bool fwd = live && band != 0u;
+            const int k = (live && !fwd && in) ? __builtin_ctz(in) : -1;
+            count_rows(a, tab, use_tab, s, be.i, k, be.j, (be.meta & 0x8000u) ? k : -1, lane);
+            if (a.emit_pairs) append_pairs(a, s, k == 0 && be.i != be.j, be.i, be.j, lane);
+            const unsigned long long fb = __ballot(fwd);
+            if (fb) {
+                uint32_t b0 = 0;
+                if (lane == 0) b0 = atomicAdd(a.mid_count, (uint32_t)__popcll(fb));
+                b0 = __shfl(b0, 0);
+                if (fwd) {
+                    const uint32_t idx = b0 + (uint32_t)__popcll(fb & below);
+                    if (idx < a.band_capacity) a.mid_band[idx] = be;
+                }
+            }
+        }
+    }
+    if (use_tab) ctab_flush(a, tab, lane);
+}
+
+void bfj_launch_mid(const PassArgs &a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mid, dim3(4096), dim3(BFJ_BLOCK), 0, st, a);
+}
