@@ -198,7 +198,7 @@ struct bfj_ctx {
         outbuf, cand, chunk_fill, pairs, ref_x, ref_aux, perm, label, ohist, oseg, pivots, pivot_norm, mid_rows,
         mid_q, mid_band;
     int reorder = 1;   // pack whole-dataset passes in a locality order (order.hip)
-    int mid_stage = 1; // FP8 passes: FP16 middle stage before the FP64 refine (refine.hip)
+    int mid_stage = 1; // low-precision passes: middle stage before the FP64 refine (refine.hip)
     uint32_t band_capacity = 0;
     uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
     int emit_pairs = 0;
@@ -443,8 +443,8 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
         ctx->band_capacity = (uint32_t)want;
     }
     RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
-    if (p.dtype == BFJ_FP8 && ctx->mid_stage) {
-        RC(ensure(ctx->mid_rows, nrows * BFJ_MAXDIM * 2));
+    if (p.dtype != BFJ_FP64 && ctx->mid_stage) {
+        RC(ensure(ctx->mid_rows, nrows * BFJ_MAXDIM * (p.dtype == BFJ_FP8 ? 2 : 4)));
         RC(ensure(ctx->mid_q, nrows * 8));
         RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
     }
@@ -521,8 +521,9 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.band = (BandEntry *)ctx->band.p;
     a.band_count = (uint32_t *)ctx->counters.p;
     a.band_capacity = ctx->band_capacity;
-    a.mid = (p.dtype == BFJ_FP8 && ctx->mid_stage) ? 1 : 0;
-    a.mid_rows = (uint16_t *)ctx->mid_rows.p;
+    a.mid = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? 1 : 0;
+    a.mid_fp32 = p.dtype != BFJ_FP8 ? 1 : 0;
+    a.mid_rows = (unsigned char *)ctx->mid_rows.p;
     a.mid_q = (double *)ctx->mid_q.p;
     a.mid_band = (BandEntry *)ctx->mid_band.p;
     a.mid_count = (uint32_t *)ctx->counters.p + 8;

@@ -162,9 +162,12 @@ struct PassArgs {
     BandEntry *band;
     uint32_t *band_count;
     uint32_t band_capacity;
-    // FP8 passes: a middle stage between classify and the FP64 refine (refine.hip).
+    // Low-precision passes: a middle stage between classify and the FP64 refine
+    // (refine.hip) on higher-precision copies of the rows: FP16 for FP8 passes, FP32
+    // for FP16/BF16 passes.
     int32_t mid;                       // 1: run it
-    uint16_t *mid_rows;                // [row][BFJ_MAXDIM] FP16 copy of the kernel-space rows, by position
+    int32_t mid_fp32;                  // the copy is FP32 (else FP16)
+    unsigned char *mid_rows;           // [row][BFJ_MAXDIM] copy of the kernel-space rows, by position
     double *mid_q;                     // [row] radius of that copy
     BandEntry *mid_band;               // records it leaves for the FP64 refine (capacity band_capacity)
     uint32_t *mid_count;

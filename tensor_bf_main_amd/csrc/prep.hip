@@ -147,11 +147,13 @@ __global__ void k_colmean(PassArgs a)
 }
 
 // ---- scale ------------------------------------------------------------------
-// FP8 passes: an FP16 copy of every kernel-space row, row-major by position
-// (BFJ_MAXDIM halves per row), and its radius ||y_i - fp16(y_i)|| bounded as in
-// k_quantize16.  The middle stage (refine.hip) decides most of the FP8 band from it,
-// reading whole 32-column chunks, so the columns from d up to a multiple of 32 are
-// written as zeros.  L lanes per row, 8 columns per lane.
+// A copy of every kernel-space row at higher precision than the join's operands,
+// row-major by position (BFJ_MAXDIM values per row), and its radius ||y_i - copy_i||
+// bounded as in k_quantize16.  The middle stage (refine.hip) decides most of the join's
+// band from it, reading whole chunks, so the columns from d up to a multiple of 32 are
+// written as zeros.  This kernel writes the FP16 copy of FP8 passes (L lanes per row, 8
+// columns per lane); k_quantize16 writes the FP32 copy of FP16/BF16 passes from the
+// values it already holds.
 __global__ __launch_bounds__(BFJ_BLOCK) void k_mid_pack(PassArgs a)
 {
     const int b = blockIdx.x;
@@ -175,22 +177,17 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid_pack(PassArgs a)
         double sq_e = 0.0, sq_y = 0.0;
         uint32_t w[4];
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            uint16_t bits[2];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int k = k0 + 2 * h + u;
-                const double v = (real && k < s.d) ? (s.x[src * s.ld + k] - mu[k]) * inv_sigma : 0.0;
-                const _Float16 q = (_Float16)v;
-                const double de = v - (double)q;
-                sq_e = fma(de, de, sq_e);
-                sq_y = fma(v, v, sq_y);
-                bits[u] = __builtin_bit_cast(uint16_t, q);
-            }
-            w[h] = (uint32_t)bits[0] | ((uint32_t)bits[1] << 16);
+        for (int j = 0; j < 8; ++j) {
+            const int k = k0 + j;
+            const double v = (real && k < s.d) ? (s.x[src * s.ld + k] - mu[k]) * inv_sigma : 0.0;
+            const _Float16 q = (_Float16)v;
+            const double de = v - (double)q;
+            const uint32_t b16 = __builtin_bit_cast(uint16_t, q);
+            w[j / 2] = (j & 1) ? (w[j / 2] | (b16 << 16)) : b16;
+            sq_e = fma(de, de, sq_e);
+            sq_y = fma(v, v, sq_y);
         }
-        if (in)
-            *reinterpret_cast<uint4 *>(a.mid_rows + (s.row_off + i) * BFJ_MAXDIM + k0) = make_uint4(w[0], w[1], w[2], w[3]);
+        if (in) *reinterpret_cast<uint4 *>(a.mid_rows + ((s.row_off + i) * BFJ_MAXDIM + k0) * 2) = make_uint4(w[0], w[1], w[2], w[3]);
         for (int o = L >> 1; o > 0; o >>= 1) {
             sq_e += __shfl_xor(sq_e, o);
             sq_y += __shfl_xor(sq_y, o);
@@ -404,7 +401,23 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_quantize16(PassArgs a)
                 y[j] = (real && k < s.d) ? (s.x[src * s.ld + k] - mu[k]) * inv_sigma : 0.0;
             }
         }
-        double sq_e = 0.0, sq_n = 0.0, sq_y = 0.0;
+        double sq_e = 0.0, sq_n = 0.0, sq_y = 0.0, sq_c = 0.0;
+        if (a.mid) {
+            // The middle stage's FP32 copy (units = 4 kc covers d up to a multiple of
+            // 32, zeros past d) and its cast error.
+            float f[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                f[j] = (float)y[j];
+                const double dc = y[j] - (double)f[j];
+                sq_c = fma(dc, dc, sq_c);
+            }
+            if (in && k0 + 8 <= BFJ_MAXDIM) {
+                uint4 *dst = reinterpret_cast<uint4 *>(a.mid_rows + ((s.row_off + i) * BFJ_MAXDIM + k0) * 4);
+                dst[0] = __builtin_bit_cast(uint4, bfj_f4{f[0], f[1], f[2], f[3]});
+                dst[1] = __builtin_bit_cast(uint4, bfj_f4{f[4], f[5], f[6], f[7]});
+            }
+        }
         uint32_t w[4];
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
@@ -439,12 +452,14 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_quantize16(PassArgs a)
             sq_e += __shfl_xor(sq_e, o);
             sq_n += __shfl_xor(sq_n, o);
             sq_y += __shfl_xor(sq_y, o);
+            if (a.mid) sq_c += __shfl_xor(sq_c, o);
         }
         if (m == 0 && rr + lr < BFJ_PREP_ROWS / 4 && i < s.npad) {
             const int64_t row = s.row_off + i;
             // Radius: measured cast error, plus the FP64 rounding of (x - mu), inflated
             // to cover the FP64 evaluation of the sums (as in k_quantize).
             const double q = real ? (sqrt(sq_e) + 2.3e-16 * sqrt(sq_y)) * (1.0 + 1e-9) + 1e-300 : 0.0;
+            if (a.mid) a.mid_q[row] = real ? (sqrt(sq_c) + 2.3e-16 * sqrt(sq_y)) * (1.0 + 1e-9) + 1e-300 : 0.0;
             a.qerr[row] = q;
             a.nrm[row] = sq_n;
             ((float *)a.nq)[row] = real ? 0.5f * (float)sq_n : 1e30f;
@@ -603,6 +618,6 @@ void bfj_launch_prep(const PassArgs &a, hipStream_t st)
     if (a.dtype == BFJ_FP16) hipLaunchKernelGGL(k_quantize16<false>, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     else if (a.dtype == BFJ_BF16) hipLaunchKernelGGL(k_quantize16<true>, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     else hipLaunchKernelGGL(k_quantize, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
-    if (a.mid) hipLaunchKernelGGL(k_mid_pack, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
+    if (a.mid && !a.mid_fp32) hipLaunchKernelGGL(k_mid_pack, dim3(a.n_row_blocks), dim3(BFJ_BLOCK), 0, st, a);
     hipLaunchKernelGGL(k_thresholds, dim3(a.n_row_blocks), dim3(BFJ_PREP_ROWS), 0, st, a);
 }

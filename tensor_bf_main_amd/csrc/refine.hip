@@ -597,33 +597,48 @@ void bfj_launch_finalize(const PassArgs &a, hipStream_t st)
     hipLaunchKernelGGL(k_finalize, dim3(a.n_row_blocks), dim3(BFJ_PREP_ROWS), 0, st, a);
 }
 
-// ---- FP8 middle stage -----------------------------------------------------------------
-// e4m3 keeps 3 mantissa bits, so the FP8 join's radius q_i is ~3.6 % of a row's norm
-// and its band holds orders of magnitude more pairs than FP16's.  Before the FP64 refine,
-// each FP8 band record is decided again from FP16 copies of its two rows (k_mid_pack,
-// 512 bytes per row instead of 2 KiB of FP64 rows): S = ||x16_i - x16_j||^2 in FP64
-// (each difference of two FP16 values is exact in FP64; the sum's rounding is far
-// below the 2^-40 margin taken), and |dist(x16_i, x16_j) - dist(y_i, y_j)| <=
-// q16_i + q16_j bounds the exact distance as in k_classify, with a radius ~128x
-// smaller.  Certain pairs are counted here; the rest go to the FP64 refine unchanged.
+// ---- middle stage ---------------------------------------------------------------------
+// The join's band is set by its operands' precision: e4m3 keeps 3 mantissa bits (a
+// radius q_i of ~3.6 % of a row's norm), BF16 8 and FP16 11.  Before the FP64 refine,
+// each band record is decided again from copies of its two rows at higher precision
+// (k_mid_pack): FP16 for FP8 passes (512 bytes per row at d = 256 instead of 2 KiB of
+// FP64), FP32 for FP16/BF16 passes (half the FP64 bytes).  S = ||c_i - c_j||^2 is
+// summed in FP64: each difference and square is within 2^-52 relative of exact (exact
+// for FP16), so the sum is within (d + 3) 2^-53 S, far inside the 2^-40 margin taken;
+// and |dist(c_i, c_j) - dist(y_i, y_j)| <= qc_i + qc_j bounds the exact distance as in
+// k_classify, with a radius 2^-8 (FP16 over e4m3) to 2^-13 (FP32 over FP16) of the
+// join's.  Certain pairs are counted here; the rest go to the FP64 refine unchanged.
 // The same runs of records per wavefront and count table as k_refine.
-// sum of (p_k - q_k)^2 over 4 units (32 FP16 values) of both rows, in FP64 (exact
-// differences).
+
+// Sum of (p_k - q_k)^2 over one chunk of both rows (4 16-byte units: 32 FP16 or 16
+// FP32 values), in FP64.
+template <bool F32>
 __device__ inline double mid_chunk(const uint4 (&p)[4], const uint4 (&q)[4], double S)
 {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const bfj_h8 hp = __builtin_bit_cast(bfj_h8, p[u]);
-        const bfj_h8 hq = __builtin_bit_cast(bfj_h8, q[u]);
+        if (F32) {
+            const bfj_f4 fp = __builtin_bit_cast(bfj_f4, p[u]);
+            const bfj_f4 fq = __builtin_bit_cast(bfj_f4, q[u]);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const double dk = (double)hp[k] - (double)hq[k];
-            S = fma(dk, dk, S);
+            for (int k = 0; k < 4; ++k) {
+                const double dk = (double)fp[k] - (double)fq[k];
+                S = fma(dk, dk, S);
+            }
+        } else {
+            const bfj_h8 hp = __builtin_bit_cast(bfj_h8, p[u]);
+            const bfj_h8 hq = __builtin_bit_cast(bfj_h8, q[u]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double dk = (double)hp[k] - (double)hq[k];
+                S = fma(dk, dk, S);
+            }
         }
     }
     return S;
 }
 
+template <bool F32>
 __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
@@ -650,13 +665,15 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
             uint32_t in = 0, band = 0;
             if (live) {
                 const SetScalars &sc = a.scal[be.meta >> 16];
-                const uint4 *p = reinterpret_cast<const uint4 *>(a.mid_rows + (s.row_off + be.i) * BFJ_MAXDIM);
-                const uint4 *q = reinterpret_cast<const uint4 *>(a.mid_rows + (s.row_off + be.j) * BFJ_MAXDIM);
-                // Chunks of 4 units (32 columns) through two register buffers, the next
-                // chunk's loads in flight while the current one is summed; k_mid_pack
-                // wrote the columns past d up to a multiple of 32 as zeros.
+                constexpr int ES = F32 ? 4 : 2; // bytes per value of the copy
+                const uint4 *p = reinterpret_cast<const uint4 *>(a.mid_rows + (s.row_off + be.i) * BFJ_MAXDIM * ES);
+                const uint4 *q = reinterpret_cast<const uint4 *>(a.mid_rows + (s.row_off + be.j) * BFJ_MAXDIM * ES);
+                // Chunks of 4 units (32 FP16 or 16 FP32 columns) through two register
+                // buffers, the next chunk's loads in flight while the current one is
+                // summed; k_mid_pack wrote the columns past d up to a multiple of 32 as
+                // zeros.
                 double S = 0.0;
-                const int nc = (s.d + 31) / 32;
+                const int nc = F32 ? (s.d + 15) / 16 : (s.d + 31) / 32;
                 uint4 pa[4], qa[4], pb[4], qb[4];
                 auto load = [&](int c, uint4 (&pp)[4], uint4 (&qq)[4]) {
 #pragma unroll
@@ -668,10 +685,10 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
                 load(0, pa, qa);
                 for (int c = 0; c < nc; c += 2) {
                     if (c + 1 < nc) load(c + 1, pb, qb);
-                    S = mid_chunk(pa, qa, S);
+                    S = mid_chunk<F32>(pa, qa, S);
                     if (c + 1 >= nc) break;
                     if (c + 2 < nc) load(c + 2, pa, qa);
-                    S = mid_chunk(pb, qb, S);
+                    S = mid_chunk<F32>(pb, qb, S);
                 }
                 const double r = (a.mid_q[s.row_off + be.i] + a.mid_q[s.row_off + be.j]) * (1.0 + 0x1p-50);
                 decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, S * (1.0 - 0x1p-40), S * (1.0 + 0x1p-40), in, band);
@@ -699,5 +716,6 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
 
 void bfj_launch_mid(const PassArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mid, dim3(4096), dim3(BFJ_BLOCK), 0, st, a);
+    if (a.mid_fp32) hipLaunchKernelGGL(k_mid<true>, dim3(4096), dim3(BFJ_BLOCK), 0, st, a);
+    else hipLaunchKernelGGL(k_mid<false>, dim3(4096), dim3(BFJ_BLOCK), 0, st, a);
 }
