@@ -101,7 +101,7 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
     t_main = (time.perf_counter() - t0) / args.steps
     pairs = float(n) * float(n)
     log(f"{dtype}: {t_main * 1e3:.1f} ms/step, join kernel {np.mean(join_ms):.1f} ms, "
-        f"refine {last.stats['ms_refine']:.1f} ms, band {last.stats['band_pairs']}")
+        f"refine {last.stats['ms_refine']:.1f} ms, band {last.stats['band_pairs']}, FP64 {last.stats['fp64_pairs']}")
     out = {
         "metric": metric, "value": pairs / t_main, "unit": "pair-distances/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_main * 1e3, "higher_is_better": True,
@@ -111,6 +111,7 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
         "total_pairs_within_eps": [int(t) for t in last.totals],
         "band_pairs": int(last.stats["band_pairs"]),
         "band_fraction": last.stats["band_pairs"] / pairs,
+        "fp64_refine_pairs": int(last.stats["fp64_pairs"]),
         "max_abs_d2_err": last.stats["max_abs_d2_err"],
         "max_band_halfwidth": last.stats["max_band_halfwidth"],
         "stage_ms": {k: last.stats[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
@@ -198,7 +199,7 @@ def batch_workload(args, ctx):
                       "dims": sorted({int(s["data"].shape[1]) for s in sets})},
            "per_dataset_launches_ms": t_each * 1e3, "per_dataset_pairs_per_s": pairs / t_each,
            "graph_speedup_vs_per_dataset": t_each / t_graph, "plan_create_s": t_plan,
-           "band_pairs": int(st["band_pairs"]),
+           "band_pairs": int(st["band_pairs"]), "fp64_refine_pairs": int(st["fp64_pairs"]),
            "stage_ms": {k: st[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
            "roofline": roofline(flops, st["ms_join"], args.dtype), "mismatches_graph_vs_eager": mismatches}
     if not args.skip_cpu:

@@ -63,7 +63,7 @@ typedef enum bfj_dtype {
 
 typedef struct bfj_stats {
     uint64_t pairs;          /* ordered pairs evaluated: (row_end - row_begin) * n, summed over datasets */
-    uint64_t band_pairs;     /* pairs re-evaluated in FP64 (union over eps) */
+    uint64_t band_pairs;     /* pairs in the join's error band for some eps (union over eps) */
     uint64_t band_capacity;  /* refine list capacity used for the pass */
     int32_t band_retries;    /* passes repeated because the refine list overflowed */
     int32_t tile_rows;       /* query rows held per wavefront in the chosen tile */
@@ -73,8 +73,10 @@ typedef struct bfj_stats {
     double max_band_halfwidth; /* widest (H - L)/2 over rows and eps, input units */
     double ms_prep;          /* device time: centre, cast, norms, thresholds */
     double ms_join;          /* device time: MFMA join with fused epsilon epilogue */
-    double ms_refine;        /* device time: FP64 band refine + count finalisation */
+    double ms_refine;        /* device time: band tests, middle stage, FP64 refine, count finalisation */
     double ms_total;         /* device time of the whole pass */
+    uint64_t fp64_pairs;     /* band pairs re-evaluated in FP64: those the middle stage (FP16/BF16/FP8
+                                passes) left undecided; band_pairs for FP64 passes */
 } bfj_stats;
 
 /* One dataset of a batch.  data/counts are host or device pointers depending on

@@ -54,6 +54,7 @@ class Stats(ctypes.Structure):
         ("ms_join", ctypes.c_double),
         ("ms_refine", ctypes.c_double),
         ("ms_total", ctypes.c_double),
+        ("fp64_pairs", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -652,8 +652,8 @@ static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>
     int retries = 0;
     for (;;) {
         RC(enqueue_pass(ctx, p, timed));
-        uint32_t ctr[2] = {0, 0};
-        HC(hipMemcpyAsync(ctr, ctx->counters.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        uint32_t ctr[9] = {}; // [0] band records, [1] candidate chunks, [8] middle-stage leftovers
+        HC(hipMemcpyAsync(ctr, ctx->counters.p, sizeof ctr, hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
         const uint32_t band_count = ctr[0];
         bool again = false;
@@ -671,6 +671,7 @@ static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>
                 memset(stats, 0, sizeof *stats);
                 stats->pairs = p.pairs;
                 stats->band_pairs = band_count;
+                stats->fp64_pairs = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? ctr[8] : band_count;
                 stats->band_capacity = ctx->band_capacity;
                 stats->band_retries = retries;
                 stats->tile_rows = p.max_rows;
@@ -1083,8 +1084,8 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
     if (pl->captured_gen != g_buf_gen.load()) RC(capture(pl));
     for (int attempt = 0; attempt < 5; ++attempt) {
         HC(hipGraphLaunch(pl->exec, ctx->stream));
-        uint32_t ctr[2] = {0, 0};
-        HC(hipMemcpyAsync(ctr, ctx->counters.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+        uint32_t ctr[9] = {}; // [0] band records, [1] candidate chunks, [8] middle-stage leftovers
+        HC(hipMemcpyAsync(ctr, ctx->counters.p, sizeof ctr, hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
         const uint32_t band_count = ctr[0];
         bool again = false;
@@ -1100,6 +1101,7 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
                 memset(stats, 0, sizeof *stats);
                 stats->pairs = pl->pass.pairs;
                 stats->band_pairs = band_count;
+                stats->fp64_pairs = (pl->pass.dtype != BFJ_FP64 && ctx->mid_stage) ? ctr[8] : band_count;
                 stats->band_capacity = ctx->band_capacity;
                 stats->band_retries = attempt;
                 stats->tile_rows = pl->pass.max_rows;

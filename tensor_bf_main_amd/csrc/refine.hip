@@ -662,34 +662,44 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
             be.meta = 0;
             if (live) be = a.band[t];
             const SetDesc &s = a.sets[be.meta >> 16];
+            // S = ||c_i - c_j||^2 of the 64 records in 8 rounds of 8.  Each record's two
+            // rows are read by a group of 8 lanes, lane h taking the row's 16-byte units h,
+            // h + 8, ..., so one load covers 128 contiguous bytes of each of 8 rows (one
+            // lane per record would touch 64 cache lines per load).  The group's partial
+            // sums meet by butterfly and go back to the record's own lane.  k_mid_pack and
+            // k_quantize16 wrote the columns past d up to a whole unit count as zeros.
+            constexpr int ES = F32 ? 4 : 2; // bytes per value of the copy
+            const int grp = lane >> 3, h = lane & 7;
+            double S = 0.0;
+            for (int b = 0; b < 8; ++b) {
+                const int src = 8 * b + grp;
+                const uint32_t ri = __shfl(be.i, src), rj = __shfl(be.j, src), rm = __shfl(be.meta, src);
+                double Sg = 0.0;
+                if (base + src < end) {
+                    const SetDesc &rs = a.sets[rm >> 16];
+                    const int units = F32 ? (rs.d + 15) / 16 * 4 : (rs.d + 31) / 32 * 4;
+                    const uint4 *p = reinterpret_cast<const uint4 *>(a.mid_rows + (rs.row_off + ri) * BFJ_MAXDIM * ES);
+                    const uint4 *q = reinterpret_cast<const uint4 *>(a.mid_rows + (rs.row_off + rj) * BFJ_MAXDIM * ES);
+                    for (int u0 = h; u0 < units; u0 += 32) {
+                        uint4 pp[4], qq[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int u = u0 + 8 * k;
+                            pp[k] = u < units ? p[u] : make_uint4(0u, 0u, 0u, 0u);
+                            qq[k] = u < units ? q[u] : make_uint4(0u, 0u, 0u, 0u);
+                        }
+                        Sg = mid_chunk<F32>(pp, qq, Sg);
+                    }
+                }
+                Sg += __shfl_xor(Sg, 1);
+                Sg += __shfl_xor(Sg, 2);
+                Sg += __shfl_xor(Sg, 4);
+                const double Sb = __shfl(Sg, 8 * h);
+                if (grp == b) S = Sb;
+            }
             uint32_t in = 0, band = 0;
             if (live) {
                 const SetScalars &sc = a.scal[be.meta >> 16];
-                constexpr int ES = F32 ? 4 : 2; // bytes per value of the copy
-                const uint4 *p = reinterpret_cast<const uint4 *>(a.mid_rows + (s.row_off + be.i) * BFJ_MAXDIM * ES);
-                const uint4 *q = reinterpret_cast<const uint4 *>(a.mid_rows + (s.row_off + be.j) * BFJ_MAXDIM * ES);
-                // Chunks of 4 units (32 FP16 or 16 FP32 columns) through two register
-                // buffers, the next chunk's loads in flight while the current one is
-                // summed; k_mid_pack wrote the columns past d up to a multiple of 32 as
-                // zeros.
-                double S = 0.0;
-                const int nc = F32 ? (s.d + 15) / 16 : (s.d + 31) / 32;
-                uint4 pa[4], qa[4], pb[4], qb[4];
-                auto load = [&](int c, uint4 (&pp)[4], uint4 (&qq)[4]) {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        pp[u] = p[4 * c + u];
-                        qq[u] = q[4 * c + u];
-                    }
-                };
-                load(0, pa, qa);
-                for (int c = 0; c < nc; c += 2) {
-                    if (c + 1 < nc) load(c + 1, pb, qb);
-                    S = mid_chunk<F32>(pa, qa, S);
-                    if (c + 1 >= nc) break;
-                    if (c + 2 < nc) load(c + 2, pa, qa);
-                    S = mid_chunk<F32>(pb, qb, S);
-                }
                 const double r = (a.mid_q[s.row_off + be.i] + a.mid_q[s.row_off + be.j]) * (1.0 + 0x1p-50);
                 decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, S * (1.0 - 0x1p-40), S * (1.0 + 0x1p-40), in, band);
             }

@@ -43,6 +43,22 @@ def test_cli_exports_reference_entry_points():
         assert sym in out, sym
 
 
+def test_stats_mirror_matches_the_header():
+    """The ctypes Stats layout is the C struct's: size and every field offset, as gcc
+    lays out bfj_stats from include/bfjoin.h."""
+    fields = [name for name, _ in bfj.Stats._fields_]
+    src = "#include <stddef.h>\n#include <stdio.h>\n#include \"bfjoin.h\"\nint main(void) {\n"
+    src += '    printf("%zu\\n", sizeof(bfj_stats));\n'
+    src += "".join(f'    printf("%zu\\n", offsetof(bfj_stats, {f}));\n' for f in fields) + "    return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as td:
+        c, exe = os.path.join(td, "layout.c"), os.path.join(td, "layout")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
+        got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(bfj.Stats)
+    assert got[1:] == [getattr(bfj.Stats, f).offset for f in fields]
+
+
 def test_validation_without_device():
     lib = bfj.load_library()
     st = bfj.Stats()

@@ -99,6 +99,10 @@ def test_workload2_reduced(ctx, dtype):
     x, eps = wl.config(2, n=20000)
     res = _check(ctx, x, eps, dtype)
     assert res.stats["band_pairs"] >= 0
+    # The middle stage decides part of a low-precision band; FP64 passes refine all of it.
+    assert 0 <= res.stats["fp64_pairs"] <= res.stats["band_pairs"]
+    if dtype == "fp64":
+        assert res.stats["fp64_pairs"] == res.stats["band_pairs"]
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
