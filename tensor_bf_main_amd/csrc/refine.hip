@@ -681,13 +681,20 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
                     const uint4 *p = reinterpret_cast<const uint4 *>(a.mid_rows + (rs.row_off + ri) * BFJ_MAXDIM * ES);
                     const uint4 *q = reinterpret_cast<const uint4 *>(a.mid_rows + (rs.row_off + rj) * BFJ_MAXDIM * ES);
                     for (int u0 = h; u0 < units; u0 += 32) {
+                        // Unconditional loads (the index clamped to the row's last unit)
+                        // so that all eight are in flight together; units past the row
+                        // are then zeroed.
                         uint4 pp[4], qq[4];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) {
                             const int u = u0 + 8 * k;
-                            pp[k] = u < units ? p[u] : make_uint4(0u, 0u, 0u, 0u);
-                            qq[k] = u < units ? q[u] : make_uint4(0u, 0u, 0u, 0u);
+                            const int uc = u < units ? u : units - 1;
+                            pp[k] = p[uc];
+                            qq[k] = q[uc];
                         }
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (u0 + 8 * k >= units) pp[k] = qq[k] = make_uint4(0u, 0u, 0u, 0u);
                         Sg = mid_chunk<F32>(pp, qq, Sg);
                     }
                 }
