@@ -103,6 +103,8 @@ def test_workload2_reduced(ctx, dtype):
     assert 0 <= res.stats["fp64_pairs"] <= res.stats["band_pairs"]
     if dtype == "fp64":
         assert res.stats["fp64_pairs"] == res.stats["band_pairs"]
+    elif res.stats["band_pairs"] > 100:
+        assert res.stats["fp64_pairs"] < res.stats["band_pairs"]
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
