@@ -35,7 +35,9 @@
 extern "C" {
 #endif
 
-#define BFJ_ABI_VERSION 1
+/* 2: bfj_stats gained fp64_pairs (a larger struct: callers built against version 1 must
+   be rebuilt). */
+#define BFJ_ABI_VERSION 2
 
 typedef struct bfj_ctx bfj_ctx;
 typedef struct bfj_plan bfj_plan;

@@ -75,6 +75,7 @@ class Dataset(ctypes.Structure):
 
 
 _lib = None
+ABI_VERSION = 2  # BFJ_ABI_VERSION of include/bfjoin.h that Stats and the prototypes mirror
 
 
 def load_library(path: str = LIB_PATH):
@@ -116,6 +117,9 @@ def load_library(path: str = LIB_PATH):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.bfj_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{path} has C ABI version {lib.bfj_abi_version()}; this module is written for "
+                           f"{ABI_VERSION}: rebuild the library")
     _lib = lib
     return lib
 
