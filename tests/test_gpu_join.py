@@ -85,6 +85,27 @@ def test_exact_boundary_pairs(ctx, dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+def test_one_ulp_either_side_of_a_pair(ctx, dtype):
+    """eps at a pair's exact FP64 distance and one ulp below it: the pair counts for the
+    first and not for the second.  Every such pair falls inside the low-precision joins'
+    band and inside the middle stage's radius, so only the FP64 refine can decide it."""
+    x = wl.gmm(2500, 96, 8, 0.05, 23)
+    rng = np.random.default_rng(8)
+    i = rng.integers(0, 2500, 4)
+    j = (i + 1 + rng.integers(0, 2499, 4)) % 2500
+    eps = []
+    for v in oracle.pair_d2(x, i, j):
+        e = float(np.sqrt(v))
+        eps += [e, float(np.nextafter(e, 0.0))]
+    res = _check(ctx, x, eps, dtype)
+    # The two eps really straddle the pair: its row holds it at eps (sqrt(d2) == eps) and
+    # not one ulp below, so its count drops (counts are in the caller's eps order).
+    for k in range(4):
+        assert res.counts[2 * k, i[k]] > res.counts[2 * k + 1, i[k]]
+    assert res.stats["fp64_pairs"] >= 1
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_row_range_and_stride(ctx, dtype):
     x0 = wl.uniform(1700, 40, 9)
     wide = np.zeros((1700, 57))
