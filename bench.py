@@ -9,7 +9,12 @@ the measured value; the plain FP64-MFMA join runs on the same data for the
 speedup, and the two must give identical per-point counts (plus seeded rows
 against the CPU oracle).
 
-Other workloads (one JSON line each, same keys):
+The headline run also measures the other configurations at their stated sizes and
+adds each as a sub-record under "configs" (--no-sub-configs: headline only).  Every
+record carries "errors": max |d2_approx - d2_fp64| over the pass's candidate pairs and
+over seeded tiles of all pairs, checked against the certified error interval.
+
+Other workloads alone (one JSON line each, same keys):
   --workload 2   n=100,000 d=128 uniform, FP64-MFMA join (exact vs oracle rows)
   --workload 4   n=500,000 d=256 unit-norm rows with 5% near-duplicates, --dtype bf16|fp8|fp16,
                  compared with the FP64-MFMA join
@@ -78,7 +83,21 @@ def cpu_baseline(x, eps, rows_n, got_counts, seed=4242):
                                       f"(OpenMP), {tc:.3g} s"}, mism
 
 
-def single_dataset(args, ctx, x, eps, dtype, metric, workload):
+def pass_errors(ctx, max_candidates=1 << 24, sample_tiles=4096):
+    """Accuracy of the pass just run (outside the timed region): max |d2_approx - d2_fp64|
+    over its candidate records (every k-th chunk beyond max_candidates) and over seeded
+    tiles of all pairs, the two bound ratios, and the pairs outside the certified interval."""
+    e = ctx.pass_errors(max_candidates=max_candidates, sample_tiles=sample_tiles, seed=2024)
+    return {"max_abs_d2_err_candidates": e["cand_max_abs_err"], "max_abs_d2_err_sampled": e["tile_max_abs_err"],
+            "candidates_checked": int(e["cand_pairs"]), "candidates_total": int(e["cand_total"]),
+            "sampled_pairs": int(e["tile_pairs"]), "bound_violations": int(e["violations"]),
+            "max_accumulation_bound_ratio": e["max_acc_ratio"], "max_quantisation_bound_ratio": e["max_quant_ratio"]}
+
+
+def single_dataset(args, ctx, x, eps, dtype, metric, workload, fp64_counts=None):
+    """One dataset: timed steps of ``dtype``, then (untimed) the accuracy statistics, the
+    host-buffer rate, the FP64-MFMA join on the same rows (or ``fp64_counts`` from an
+    earlier run of it) and the CPU oracle on a row sample."""
     n, d = x.shape
     e = np.atleast_1d(eps)
     xd = ctx.to_device(x)  # rows resident in HBM before the timed region
@@ -112,7 +131,7 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
         "band_pairs": int(last.stats["band_pairs"]),
         "band_fraction": last.stats["band_pairs"] / pairs,
         "fp64_refine_pairs": int(last.stats["fp64_pairs"]),
-        "max_abs_d2_err": last.stats["max_abs_d2_err"],
+        "candidates": int(last.stats["candidates"]),
         "max_band_halfwidth": last.stats["max_band_halfwidth"],
         "stage_ms": {k: last.stats[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
         # Kernel-only time of a pass (prep + join + classify/refine/finalize, HIP events),
@@ -125,6 +144,7 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
     flops = 2.0 * d * n * (n + 1) / 2
     out["roofline"] = roofline(flops, float(np.mean(join_ms)), dtype,
                                None if args.n else measured_traffic(args.workload, dtype, int(last.stats["tile_rows"])))
+    out["errors"] = pass_errors(ctx)
     got = c_main.numpy()
     # The host-buffer entry point (rows in host memory, one H2D copy per call, counts
     # copied back): the PCIe-inclusive rate, reported beside the value, never as it.
@@ -136,7 +156,10 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
     out["host_buffers_pairs_per_s"] = pairs / t_host
     host_mism = int((host_res.counts != got).sum())
     mismatches = host_mism
-    if dtype != "fp64" and not args.skip_fp64:
+    out["fp64_counts"] = None
+    if dtype != "fp64" and fp64_counts is not None:
+        mismatches += int((got != fp64_counts).sum())
+    elif dtype != "fp64" and not args.skip_fp64:
         c64 = ctx.empty((e.size, n), np.int64)
         log("fp64 warmup")
         step("fp64", c64)
@@ -145,7 +168,8 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
         for _ in range(args.fp64_steps):
             r64 = step("fp64", c64)
         t64 = (time.perf_counter() - t0) / args.fp64_steps
-        mismatches += int((got != c64.numpy()).sum()) + int((r64.totals != last.totals).sum())
+        out["fp64_counts"] = c64.numpy()
+        mismatches += int((got != out["fp64_counts"]).sum()) + int((r64.totals != last.totals).sum())
         out["fp64_ms_per_step"] = t64 * 1e3
         out["fp64_pairs_per_s"] = pairs / t64
         out["speedup_vs_fp64"] = t64 / t_main
@@ -158,6 +182,67 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload):
         out["oracle_sample_mismatches"] = sm
         mismatches = (mismatches or 0) + sm
     out["mismatches"] = mismatches
+    xd.free()
+    c_main.free()
+    return out
+
+
+def sub_configs(args, ctx):
+    """The other configurations at their full sizes, each a sub-record of the headline
+    line with the same measurements (timed steps, stage times, roofline, accuracy,
+    mismatches against the FP64 path and the CPU oracle on a row sample)."""
+    from tensor_bf_main_amd import workloads as wl
+
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.n, sub.cpu_rows, sub.skip_cpu = args.sub_steps, 1, 0, 0, False
+    out = {}
+
+    def run(key, fn):
+        t0 = time.perf_counter()
+        try:
+            rec = fn()
+        except Exception as exc:  # a failing sub-record must not hide the headline
+            rec = {"error": f"{type(exc).__name__}: {exc}"}
+        rec.pop("fp64_counts", None)
+        rec["wall_s"] = time.perf_counter() - t0
+        for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data", "unit"):
+            rec.pop(k, None)
+        out[key] = rec
+        log(f"{key}: {json.dumps({k: rec.get(k) for k in ('ms_per_step', 'value', 'mismatches', 'error')})}")
+
+    def c1():
+        x, eps = wl.config(1)
+        sub.workload = 1
+        return single_dataset(sub, ctx, x, eps, "fp64", "", "uniform_n10k_d32")
+
+    def c2():
+        x, eps = wl.config(2)
+        sub.workload, sub.cpu_rows = 2, 2000
+        return single_dataset(sub, ctx, x, eps, "fp64", "", "uniform")
+
+    x4, e4 = wl.config(4)
+    keep = {}
+
+    def c4(dtype):
+        def f():
+            sub.workload, sub.cpu_rows, sub.skip_fp64 = 4, 400, False
+            rec = single_dataset(sub, ctx, x4, e4, dtype, "", "unit_norm_5pct_dups", keep.get("fp64"))
+            if rec.get("fp64_counts") is not None:
+                keep["fp64"] = rec["fp64_counts"]
+            return rec
+        return f
+
+    run("1_fp64_n10k_d32", c1)
+    run("2_fp64_n100k_d128", c2)
+    run("4_bf16_n500k_d256", c4("bf16"))
+    run("4_fp8_n500k_d256", c4("fp8"))
+    del x4, keep
+
+    def c5():
+        sub.workload, sub.dtype = 5, "fp16"
+        return batch_workload(sub, ctx)
+
+    run("5_fp16_batch512", c5)
     return out
 
 
@@ -180,6 +265,7 @@ def batch_workload(args, ctx):
     for _ in range(args.steps):
         st = plan.run()
     t_graph = (time.perf_counter() - t0) / args.steps
+    errors = pass_errors(ctx)  # the plan's last replay
     graph_counts = [dv["counts"].numpy() for dv in dev]
     log(f"graph: {t_graph * 1e3:.1f} ms/step ({pairs / t_graph:.3e} pairs/s)")
     # The same work as one call per dataset (eager, per-dataset launches + sync).
@@ -200,20 +286,28 @@ def batch_workload(args, ctx):
            "per_dataset_launches_ms": t_each * 1e3, "per_dataset_pairs_per_s": pairs / t_each,
            "graph_speedup_vs_per_dataset": t_each / t_graph, "plan_create_s": t_plan,
            "band_pairs": int(st["band_pairs"]), "fp64_refine_pairs": int(st["fp64_pairs"]),
+           "candidates": int(st["candidates"]), "join_counted": int(st["join_counted"]),
            "stage_ms": {k: st[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
-           "roofline": roofline(flops, st["ms_join"], args.dtype), "mismatches_graph_vs_eager": mismatches}
+           "roofline": roofline(flops, st["ms_join"], args.dtype), "errors": errors,
+           "mismatches_graph_vs_eager": mismatches}
     if not args.skip_cpu:
         from oracle import oracle
 
-        # Exact counts of a few whole datasets against the oracle.
-        pick = np.random.default_rng(7).choice(len(sets), min(6, len(sets)), replace=False)
+        # Exact counts of a seeded few whole datasets against the oracle (among those of
+        # at most 8192 rows, so that the check stays within seconds).
+        small = [k for k, s in enumerate(sets) if s["data"].shape[0] <= 8192]
+        pick = np.random.default_rng(7).choice(small, min(6, len(small)), replace=False) if small else []
         mism = 0
         for k in pick:
             oc, _ = oracle.join(sets[k]["data"], sets[k]["eps"])
             mism += int((graph_counts[k] != oc).sum())
-        out["oracle_sets_checked"] = len(pick)
+        out["oracle_sets_checked"] = [int(k) for k in pick]
         out["oracle_mismatches"] = mism
+    out["mismatches"] = mismatches + out.get("oracle_mismatches", 0)
     plan.close()
+    for dv in dev:
+        dv["data"].free()
+        dv["counts"].free()
     return out
 
 
@@ -252,6 +346,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0, help="query rows timed on the CPU (0 = auto, ~10-30 s)")
     ap.add_argument("--tile-rows", type=int, default=0, help="force the join's rows per wavefront (0 = library default)")
+    ap.add_argument("--sub-steps", type=int, default=3, help="timed steps of each other configuration")
+    ap.add_argument("--no-sub-configs", action="store_true",
+                    help="headline only (by default workload 3 also measures workloads 1, 2, 4 and 5)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (scales HBM byte counters)")
     args = ap.parse_args()
@@ -290,6 +387,11 @@ def main():
                      "unit_norm_5pct_dups")}
         metric, wname = names[args.workload]
         out = single_dataset(args, ctx, x, eps, dtype, metric, wname)
+        out.pop("fp64_counts", None)
+        del x
+        if args.workload == 3 and not args.n and not args.no_sub_configs:
+            # Every other configuration at its stated size, measured in the same run.
+            out["configs"] = sub_configs(args, ctx)
     ctx.close()
     print(json.dumps(out), flush=True)
 

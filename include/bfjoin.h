@@ -35,9 +35,10 @@
 extern "C" {
 #endif
 
-/* 2: bfj_stats gained fp64_pairs (a larger struct: callers built against version 1 must
-   be rebuilt). */
-#define BFJ_ABI_VERSION 2
+/* 2: bfj_stats gained fp64_pairs.  3: bfj_stats gained candidates and join_counted;
+   bfj_set_capacity, bfj_pass_errors.  (A larger struct: callers built against an
+   earlier version must be rebuilt.) */
+#define BFJ_ABI_VERSION 3
 
 typedef struct bfj_ctx bfj_ctx;
 typedef struct bfj_plan bfj_plan;
@@ -71,7 +72,6 @@ typedef struct bfj_stats {
     int32_t tile_rows;       /* query rows held per wavefront in the chosen tile */
     int32_t k_padded;        /* dot-product length after zero padding */
     int32_t dtype;
-    double max_abs_d2_err;   /* max |d2_approx - d2_fp64| over refined pairs, input units */
     double max_band_halfwidth; /* widest (H - L)/2 over rows and eps, input units */
     double ms_prep;          /* device time: centre, cast, norms, thresholds */
     double ms_join;          /* device time: MFMA join with fused epsilon epilogue */
@@ -79,6 +79,10 @@ typedef struct bfj_stats {
     double ms_total;         /* device time of the whole pass */
     uint64_t fp64_pairs;     /* band pairs re-evaluated in FP64: those the middle stage (FP16/BF16/FP8
                                 passes) left undecided; band_pairs for FP64 passes */
+    uint64_t candidates;     /* pair records the join streamed to the band tests (pairs it could
+                                neither reject nor count itself) */
+    uint64_t join_counted;   /* pairs the join counted itself (certain for every eps), 0 when the
+                                pass streams every candidate */
 } bfj_stats;
 
 /* One dataset of a batch.  data/counts are host or device pointers depending on
@@ -93,6 +97,22 @@ typedef struct bfj_dataset {
     int64_t *counts;    /* [n_eps][n], self excluded; may be NULL */
     uint64_t *totals;   /* [n_eps], self included; may be NULL */
 } bfj_dataset;
+
+/* Accuracy of a pass's approximate squared distances (bfj_pass_errors). */
+typedef struct bfj_errors {
+    uint64_t cand_total;      /* candidate records the pass streamed */
+    uint64_t cand_pairs;      /* of those, checked here */
+    uint64_t tile_pairs;      /* pairs checked in the sampled tiles */
+    uint64_t violations;      /* checked pairs whose FP64 distance lies outside the interval the
+                                 band tests assumed for them: 0 when the bound holds */
+    double cand_max_abs_err;  /* max |S_hat - d2| over the candidates checked, input units */
+    double tile_max_abs_err;  /* max |S_hat - d2| over the sampled tiles, input units */
+    double max_acc_ratio;     /* sampled tiles, FP16/BF16/FP8 passes: max |S_hat - S_cast| / E_i,
+                                 S_cast the exact squared distance of the cast rows and E_i the
+                                 accumulation bound (<= 1 when it holds); -1 for FP64 passes */
+    double max_quant_ratio;   /* sampled tiles, FP16/BF16/FP8 passes: max |dist_cast - dist| /
+                                 (q_i + q_j), the quantisation radius (<= 1); -1 for FP64 passes */
+} bfj_errors;
 
 const char *bfj_last_error(void);
 int bfj_abi_version(void);
@@ -172,6 +192,27 @@ int bfj_set_tile_rows(bfj_ctx *ctx, int32_t dtype, int32_t tile_rows);
  * are all queries packs them grouped by nearest of 128 pivot rows, so the pairs near
  * eps fall into few tiles.  Results do not depend on it; enable = 0 keeps input order. */
 int bfj_set_reorder(bfj_ctx *ctx, int32_t enable);
+
+/* Capacities of the candidate stream (chunks of 1024 records) and of the band lists
+ * (records) for the next pass; 0 leaves one as it is.  No counterpart in the
+ * reference, which has neither.  A pass that needs more reports it, and the library
+ * grows the buffer and repeats the pass (stats band_retries): results never depend
+ * on these values, only the number of passes does.  Used to pre-size the buffers
+ * for a known workload, or to force the grow-and-repeat path in tests. */
+int bfj_set_capacity(bfj_ctx *ctx, uint64_t candidate_chunks, uint64_t band_records);
+
+/* Accuracy of the last pass run on this context (no counterpart in the reference,
+ * which reports no error): sigma^2 S_hat, the squared distance the pass's dot
+ * products gave, against the FP64 direct form d2 of kernel_join.cu:184-189, over
+ *   (a) the pass's candidate records (the pairs it could not reject: near or inside
+ *       eps), all of them, or every k-th chunk so that about max_candidates are
+ *       checked (0: all), and
+ *   (b) sample_tiles seeded 16 x 16 tiles of all pairs, spread over the datasets by
+ *       their pair counts and recomputed with the pass's own MFMA sequence.
+ * Every checked pair is also tested against the interval the band tests assumed.
+ * Runs after the pass, outside its timing; it needs the pass's buffers and rows,
+ * so call it before any other call on the context and before freeing the rows. */
+int bfj_pass_errors(bfj_ctx *ctx, uint64_t max_candidates, int64_t sample_tiles, uint64_t seed, bfj_errors *out);
 int bfj_tile_rows_options(int32_t dtype, int32_t d, int32_t *options, int32_t max_options, int32_t *n_options);
 
 #ifdef __cplusplus

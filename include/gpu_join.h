@@ -60,6 +60,14 @@ int GPUJoinMainBruteForceReference(unsigned int searchMode, unsigned int device,
                                    unsigned int dim, unsigned int computeDim, double epsilon, int computePrec,
                                    int accumPrec, uint64_t *totalNeighbors);
 
+/*
+ * Accuracy of the last join on `device` through this file's entry points (no
+ * counterpart in the reference): bfj_pass_errors on the device's context.  Call it
+ * right after the join.  Returns a bfj_status.
+ */
+int GPUJoinPassErrors(unsigned int device, uint64_t maxCandidates, int64_t sampleTiles, uint64_t seed,
+                      bfj_errors *errors);
+
 /* searchMode -> bfj_dtype, or -1 for modes that are errors (5, 6, unknown). */
 int searchModeDtype(unsigned int searchMode);
 

@@ -16,8 +16,12 @@ Functions and the reference code they restate (jeevana-swaroop/tensor_bf_main):
 * :func:`parse_text` -- the text reader, dataset.cu:24-38: values separated by
   ``", "`` or whitespace, grouped into rows of ``dim`` values regardless of line
   breaks, a trailing partial row dropped.
-* :func:`pad_rows` -- host padding of main.cu:39-57: rows zero-padded to
-  ``compute_dim`` and ``ADDITIONAL_POINTS`` (15) zero rows appended.
+* :func:`reference_total` -- the reference's own arithmetic of search modes 1-4
+  (kernel_join.cu:168-524), for the bug-for-bug totals.
+
+How each is pinned (tests/test_oracle.py, tests/test_reference_modes.py): golden
+vectors of the reference's own data files, exact-rational emulations, closed forms
+and invariants, and the reference's own reader compiled from its sources into _ref/.
 """
 from __future__ import annotations
 
@@ -31,7 +35,6 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libbfj_oracle.so")
 _REF_IMPORT = os.path.join(_HERE, "_ref", "ref_import")
-ADDITIONAL_POINTS = 15  # params.h:20
 
 _lib = None
 
@@ -184,14 +187,6 @@ def parse_text(text: str, dim: int) -> np.ndarray:
             pos = c.end()
     m = len(vals) // dim
     return np.asarray(vals[: m * dim], dtype=np.float64).reshape(m, dim)
-
-
-def pad_rows(rows: np.ndarray, compute_dim: int) -> np.ndarray:
-    """main.cu:39-57 restated: zero-pad to compute_dim, append ADDITIONAL_POINTS zero rows."""
-    n, d = rows.shape
-    out = np.zeros((n + ADDITIONAL_POINTS, compute_dim), dtype=np.float64)
-    out[:n, :d] = rows
-    return out
 
 
 def ref_import(path: str):

@@ -48,13 +48,31 @@ class Stats(ctypes.Structure):
         ("tile_rows", ctypes.c_int32),
         ("k_padded", ctypes.c_int32),
         ("dtype", ctypes.c_int32),
-        ("max_abs_d2_err", ctypes.c_double),
         ("max_band_halfwidth", ctypes.c_double),
         ("ms_prep", ctypes.c_double),
         ("ms_join", ctypes.c_double),
         ("ms_refine", ctypes.c_double),
         ("ms_total", ctypes.c_double),
         ("fp64_pairs", ctypes.c_uint64),
+        ("candidates", ctypes.c_uint64),
+        ("join_counted", ctypes.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Errors(ctypes.Structure):
+    """bfj_errors: accuracy of a pass's approximate squared distances."""
+    _fields_ = [
+        ("cand_total", ctypes.c_uint64),
+        ("cand_pairs", ctypes.c_uint64),
+        ("tile_pairs", ctypes.c_uint64),
+        ("violations", ctypes.c_uint64),
+        ("cand_max_abs_err", ctypes.c_double),
+        ("tile_max_abs_err", ctypes.c_double),
+        ("max_acc_ratio", ctypes.c_double),
+        ("max_quant_ratio", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -75,7 +93,7 @@ class Dataset(ctypes.Structure):
 
 
 _lib = None
-ABI_VERSION = 2  # BFJ_ABI_VERSION of include/bfjoin.h that Stats and the prototypes mirror
+ABI_VERSION = 3  # BFJ_ABI_VERSION of include/bfjoin.h that Stats and the prototypes mirror
 
 
 def load_library(path: str = LIB_PATH):
@@ -107,6 +125,8 @@ def load_library(path: str = LIB_PATH):
         "bfj_autotune": (ctypes.c_int, [vp, i32, i32, i64, ctypes.POINTER(i32)]),
         "bfj_set_tile_rows": (ctypes.c_int, [vp, i32, i32]),
         "bfj_set_reorder": (ctypes.c_int, [vp, i32]),
+        "bfj_set_capacity": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint64]),
+        "bfj_pass_errors": (ctypes.c_int, [vp, ctypes.c_uint64, i64, ctypes.c_uint64, ctypes.POINTER(Errors)]),
         "bfj_tile_rows_options": (ctypes.c_int, [i32, i32, ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]),
         "bfj_buffer_alloc": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(vp)]),
         "bfj_buffer_free": (ctypes.c_int, [vp, vp]),
@@ -201,6 +221,24 @@ class DeviceArray:
             pass
 
 
+def host_rows(data):
+    """(array, n, d, ld) of a host float64 matrix as the C ABI takes it: rows of d values at
+    a stride of ld >= d values, starting at the array's data pointer.  A view with unit
+    column stride and a positive row stride (a column slice of a wider array, every k-th
+    row) is passed as it is; anything else (reversed or broadcast rows, a column stride)
+    is copied to a contiguous array first."""
+    x = np.asarray(data)
+    if x.dtype != np.float64 or x.ndim != 2:
+        raise ValueError("data must be a 2-D float64 array")
+    n, d = x.shape
+    if n > 1 and (x.strides[1] != 8 or x.strides[0] < 8 * d or x.strides[0] % 8):
+        x = np.ascontiguousarray(x)
+    elif n <= 1 and x.strides[1] != 8 and d > 1:
+        x = np.ascontiguousarray(x)
+    ld = x.strides[0] // 8 if n > 1 else d
+    return x, n, d, ld
+
+
 def _dev_shape(x):
     """(n, d, ld, ptr) of a device matrix: a DeviceArray or any object with
     data_ptr()/shape/stride() (e.g. a torch tensor from the same HIP runtime)."""
@@ -248,13 +286,7 @@ class Context:
     def self_join(self, data, eps, dtype="fp16", row_begin: int = 0, row_end: int | None = None,
                   counts: bool = True) -> JoinResult:
         """Join a host (numpy) float64 array of shape (n, d), or a strided view with unit column stride."""
-        x = np.asarray(data)
-        if x.dtype != np.float64 or x.ndim != 2:
-            raise ValueError("data must be a 2-D float64 array")
-        if x.strides[1] != 8 or x.strides[0] % 8:
-            x = np.ascontiguousarray(x)
-        n, d = x.shape
-        ld = x.strides[0] // 8 if n > 1 else d
+        x, n, d, ld = host_rows(data)
         e = _eps_array(eps)
         if row_end is None:
             row_end = n
@@ -383,6 +415,22 @@ class Context:
 
     def set_tile_rows(self, dtype, rows: int):
         _check(self.lib.bfj_set_tile_rows(self._h, _dtype_id(dtype), rows))
+
+    def set_capacity(self, candidate_chunks: int = 0, band_records: int = 0):
+        """Capacities of the candidate stream (chunks of 1024 records) and the band lists for
+        the next pass (0 leaves one as it is).  A pass that needs more grows them and is
+        repeated, so results never depend on these."""
+        _check(self.lib.bfj_set_capacity(self._h, int(candidate_chunks), int(band_records)))
+
+    def pass_errors(self, max_candidates: int = 1 << 24, sample_tiles: int = 4096, seed: int = 12345) -> dict:
+        """Accuracy of the last pass on this context (call it right after the pass): max
+        |d2_approx - d2_fp64| over its candidate records (every k-th chunk beyond
+        ``max_candidates``, 0 = all) and over ``sample_tiles`` seeded tiles of all pairs, the
+        accumulation and quantisation bound ratios, and the pairs outside the certified
+        interval (``violations``, 0 when the bound holds)."""
+        e = Errors()
+        _check(self.lib.bfj_pass_errors(self._h, int(max_candidates), int(sample_tiles), int(seed), ctypes.byref(e)))
+        return e.as_dict()
 
     def set_reorder(self, enable: bool):
         """Turn the locality order of whole-dataset passes on (default) or off; results

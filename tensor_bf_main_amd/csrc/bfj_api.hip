@@ -182,6 +182,7 @@ static double eps_threshold(double eps)
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    uint64_t id = 0; // unique per allocation: a freed and re-made buffer never keeps its id
 };
 
 struct Group {
@@ -199,22 +200,31 @@ struct bfj_ctx {
         mid_q, mid_band;
     int reorder = 1;   // pack whole-dataset passes in a locality order (order.hip)
     int mid_stage = 1; // low-precision passes: middle stage before the FP64 refine (refine.hip)
-    uint32_t band_capacity = 0;
+    uint64_t band_capacity = 0;  // records of the band list and of the middle stage's list
     uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
     int emit_pairs = 0;
     uint32_t chunk_capacity = 0; // candidate chunks of BFJ_CHUNK records
     int tile_rows[4][33] = {}; // [dtype][kc] forced/tuned R (0 = heuristic)
     // Work items of the last plan shape (repeated calls on the same shapes reuse them,
-    // and skip the upload while the device copy is current).
+    // and skip the upload while the device copy is current).  Item lists are known by
+    // a unique id, never by address: a list freed with its plan can be replaced by a
+    // new one at the same address.
     std::vector<int64_t> item_key;
     std::shared_ptr<const std::vector<WorkItem>> item_cache;
+    uint64_t item_cache_id = 0;
     std::vector<Group> group_cache;
-    const std::vector<WorkItem> *items_staged = nullptr;
-    void *items_staged_buf = nullptr;
+    uint64_t items_staged_id = 0;     // item list whose copy the items buffer holds (0: none)
+    uint64_t items_staged_buf_id = 0; // allocation of the items buffer it was copied into
+    // The last pass that ran to completion on this context (bfj_pass_errors reads its
+    // buffers); cleared when another pass starts or its rows may be gone.
+    std::unique_ptr<struct PassPlan> last;
+    DevBuf errbuf;
 };
 
 // Bumped whenever a device buffer moves: a captured graph holds raw pointers.
 static std::atomic<uint64_t> g_buf_gen{1};
+// Source of the DevBuf and work-item list ids.
+static std::atomic<uint64_t> g_obj_id{1};
 
 static int ensure(DevBuf &b, size_t bytes)
 {
@@ -223,6 +233,7 @@ static int ensure(DevBuf &b, size_t bytes)
     if (b.p) HC(hipFree(b.p));
     b.p = nullptr;
     b.cap = 0;
+    b.id = 0;
     size_t want = bytes + bytes / 8 + 256;
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
@@ -231,6 +242,7 @@ static int ensure(DevBuf &b, size_t bytes)
         return fail(BFJ_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
     }
     b.cap = want;
+    b.id = g_obj_id.fetch_add(1);
     return BFJ_OK;
 }
 
@@ -239,6 +251,7 @@ static void release(DevBuf &b)
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+    b.id = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -256,6 +269,7 @@ struct PassPlan {
     int dtype = 0;
     std::vector<SetDesc> sets;
     std::shared_ptr<const std::vector<WorkItem>> items; // shared with the context's cache
+    uint64_t items_id = 0;                               // unique id of that list
     std::vector<Group> groups;
     std::vector<int32_t> blk_set, set_blk0, set_nblk;
     std::vector<int64_t> blk_row0;
@@ -370,6 +384,7 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
     for (auto &s : p.sets) key.insert(key.end(), {s.kc, s.npad, s.sym, s.row_begin, s.row_end});
     if (ctx->item_cache && key == ctx->item_key) {
         p.items = ctx->item_cache;
+        p.items_id = ctx->item_cache_id;
         p.groups = ctx->group_cache;
         for (auto &g : p.groups) p.max_rows = std::max(p.max_rows, g.rows);
         return BFJ_OK;
@@ -415,8 +430,10 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         if (g.n_items) p.groups.push_back(g);
     }
     p.items = items;
+    p.items_id = g_obj_id.fetch_add(1);
     ctx->item_key = key;
     ctx->item_cache = items;
+    ctx->item_cache_id = p.items_id;
     ctx->group_cache = p.groups;
     return BFJ_OK;
 }
@@ -438,10 +455,8 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->nrm, nrows * 8));
     RC(ensure(ctx->scales, nrows + 1024));
     RC(ensure(ctx->cnt, std::max<int64_t>(p.cnt_total, 1) * 4));
-    if (ctx->band_capacity == 0) {
-        const uint64_t want = std::min<uint64_t>(std::max<uint64_t>((uint64_t)p.rows_total * 64, 1u << 20), 1u << 28);
-        ctx->band_capacity = (uint32_t)want;
-    }
+    if (ctx->band_capacity == 0)
+        ctx->band_capacity = std::min<uint64_t>(std::max<uint64_t>((uint64_t)p.rows_total * 64, 1u << 20), 1u << 28);
     RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
     if (p.dtype != BFJ_FP64 && ctx->mid_stage) {
         RC(ensure(ctx->mid_rows, nrows * BFJ_MAXDIM * (p.dtype == BFJ_FP8 ? 2 : 4)));
@@ -457,7 +472,7 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     }
     RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
     RC(ensure(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
-    RC(ensure(ctx->counters, 64 * 4));
+    RC(ensure(ctx->counters, BFJ_CTR_SLOTS * 8));
     RC(ensure(ctx->colsum, std::max<size_t>(p.blk_set.size(), 1) * BFJ_MAXDIM * 8 * 3)); // sums, maxima, minima
     RC(ensure(ctx->mu, p.sets.size() * BFJ_MAXDIM * 8));
     RC(ensure(ctx->sets, p.sets.size() * sizeof(SetDesc)));
@@ -477,11 +492,12 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->pivots, std::max<size_t>(ns, 1) * BFJ_PIVOTS * BFJ_MAXDIM * 2));
     RC(ensure(ctx->pivot_norm, std::max<size_t>(ns, 1) * BFJ_PIVOTS * 4));
     HC(hipMemcpyAsync(ctx->sets.p, p.sets.data(), ns * sizeof(SetDesc), hipMemcpyHostToDevice, ctx->stream));
-    if (n_items && (ctx->items_staged != p.items.get() || ctx->items_staged_buf != ctx->items.p)) {
-        ctx->items_staged = p.items.get();
-        ctx->items_staged_buf = ctx->items.p;
+    if (n_items && (ctx->items_staged_id != p.items_id || ctx->items_staged_buf_id != ctx->items.id)) {
+        ctx->items_staged_id = 0;
         HC(hipMemcpyAsync(ctx->items.p, p.items->data(), n_items * sizeof(WorkItem), hipMemcpyHostToDevice,
                           ctx->stream));
+        ctx->items_staged_id = p.items_id;
+        ctx->items_staged_buf_id = ctx->items.id;
     }
     unsigned char *t = (unsigned char *)ctx->tabs.p;
     HC(hipMemcpyAsync(t, p.blk_row0.data(), nb * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -518,22 +534,24 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.scales = (uint8_t *)ctx->scales.p;
     a.cnt = (uint32_t *)ctx->cnt.p;
     a.cnt_len = p.cnt_total;
+    unsigned long long *ctr = (unsigned long long *)ctx->counters.p;
     a.band = (BandEntry *)ctx->band.p;
-    a.band_count = (uint32_t *)ctx->counters.p;
+    a.band_count = ctr + BFJ_CTR_BAND;
     a.band_capacity = ctx->band_capacity;
     a.mid = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? 1 : 0;
     a.mid_fp32 = p.dtype != BFJ_FP8 ? 1 : 0;
     a.mid_rows = (unsigned char *)ctx->mid_rows.p;
     a.mid_q = (double *)ctx->mid_q.p;
     a.mid_band = (BandEntry *)ctx->mid_band.p;
-    a.mid_count = (uint32_t *)ctx->counters.p + 8;
+    a.mid_count = ctr + BFJ_CTR_MID;
     a.cand = (const Candidate *)ctx->cand.p;
     a.chunk_fill = (const uint32_t *)ctx->chunk_fill.p;
-    a.chunk_count = (const uint32_t *)ctx->counters.p + 1;
+    a.chunk_count = ctr + BFJ_CTR_CHUNKS;
     a.chunk_capacity = ctx->chunk_capacity;
+    a.cand_records = ctr + BFJ_CTR_CANDS;
     a.emit_pairs = ctx->emit_pairs;
     a.pairs = (uint2 *)ctx->pairs.p;
-    a.pair_count = (unsigned long long *)((uint32_t *)ctx->counters.p + 4);
+    a.pair_count = ctr + BFJ_CTR_PAIRS;
     a.pair_capacity = ctx->pair_capacity;
     int max_nblk = 1;
     for (int32_t v : p.set_nblk) max_nblk = std::max(max_nblk, (int)v);
@@ -551,16 +569,17 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
 // memset nodes: safe to capture in a graph).
 static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
 {
+    ctx->last.reset();
     const PassArgs a = pass_args(ctx, p);
     hipStream_t st = ctx->stream;
     if (timed) HC(hipEventRecord(ctx->ev[0], st));
     bfj_launch_zero(ctx->cnt.p, (size_t)std::max<int64_t>(p.cnt_total, 1) * 4, st);
     bfj_launch_zero(ctx->scal.p, p.sets.size() * sizeof(SetScalars), st);
-    bfj_launch_zero(ctx->counters.p, 64 * 4, st);
+    bfj_launch_zero(ctx->counters.p, BFJ_CTR_SLOTS * 8, st);
     if (p.blk_set.size()) bfj_launch_prep(a, st);
     HC(hipGetLastError());
     if (timed) HC(hipEventRecord(ctx->ev[1], st));
-    uint32_t *item_ctr = (uint32_t *)ctx->counters.p + 16;
+    uint32_t *item_ctr = (uint32_t *)((unsigned long long *)ctx->counters.p + BFJ_CTR_ITEMS);
     for (size_t gi = 0; gi < p.groups.size(); ++gi) {
         const Group &g = p.groups[gi];
         JoinArgs j;
@@ -575,7 +594,7 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
         j.scales = a.scales;
         j.cand = (Candidate *)ctx->cand.p;
         j.chunk_fill = (uint32_t *)ctx->chunk_fill.p;
-        j.chunk_count = (uint32_t *)ctx->counters.p + 1;
+        j.chunk_count = (unsigned long long *)ctx->counters.p + BFJ_CTR_CHUNKS;
         j.chunk_capacity = ctx->chunk_capacity;
         JoinLaunch L;
         L.dtype = p.dtype;
@@ -625,22 +644,72 @@ static int check_finite(const std::vector<SetScalars> &sc)
 // After a pass: if the candidate stream or the refine list overflowed (records
 // past the capacity were dropped), grow both to what the pass asked for and set
 // *again so that the caller repeats the pass.
-static int grow_on_overflow(bfj_ctx *ctx, const uint32_t ctr[2], bool *again)
+#define BFJ_MAX_CHUNKS (1ull << 26)   // 64 Mi chunks of 1024 candidate records (1 TiB): more is refused
+#define BFJ_MAX_BAND (1ull << 34)     // band-list records (256 GiB per list): more is refused
+static int grow_on_overflow(bfj_ctx *ctx, const unsigned long long *ctr, bool *again)
 {
     *again = false;
-    const uint32_t band_count = ctr[0], chunks = ctr[1];
+    const unsigned long long band_count = ctr[BFJ_CTR_BAND], chunks = ctr[BFJ_CTR_CHUNKS];
     if (chunks > ctx->chunk_capacity) {
-        ctx->chunk_capacity = (uint32_t)std::min<uint64_t>((uint64_t)chunks + chunks / 4 + 64, 1u << 26);
+        if (chunks > BFJ_MAX_CHUNKS)
+            return fail(BFJ_ERR_UNSUPPORTED, "the pass needs %llu candidate chunks (at most %llu)", chunks,
+                        BFJ_MAX_CHUNKS);
+        ctx->chunk_capacity = (uint32_t)std::min<unsigned long long>(chunks + chunks / 4 + 64, BFJ_MAX_CHUNKS);
         RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
         RC(ensure(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
         *again = true;
     }
     if (band_count > ctx->band_capacity) {
-        ctx->band_capacity = (uint32_t)std::min<uint64_t>((uint64_t)band_count + band_count / 4 + 1024, 0xffffffffull);
+        if (band_count > BFJ_MAX_BAND)
+            return fail(BFJ_ERR_UNSUPPORTED, "the pass puts %llu pairs in the band (at most %llu)", band_count,
+                        BFJ_MAX_BAND);
+        ctx->band_capacity = std::min<unsigned long long>(band_count + band_count / 4 + 1024, BFJ_MAX_BAND);
         RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
         if (ctx->mid_band.p) RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
         *again = true;
     }
+    return BFJ_OK;
+}
+
+// Totals and statistics of a finished pass (no overflow): ctr holds its counters,
+// retries the passes repeated before it.
+static int collect_pass(bfj_ctx *ctx, const PassPlan &p, const unsigned long long *ctr, int retries, bool timed,
+                        std::vector<SetScalars> &sc, bfj_stats *stats)
+{
+    sc.assign(p.sets.size(), SetScalars());
+    HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
+    RC(check_finite(sc));
+    if (!stats) return BFJ_OK;
+    memset(stats, 0, sizeof *stats);
+    stats->pairs = p.pairs;
+    stats->band_pairs = ctr[BFJ_CTR_BAND];
+    stats->fp64_pairs = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? ctr[BFJ_CTR_MID] : ctr[BFJ_CTR_BAND];
+    stats->candidates = ctr[BFJ_CTR_CANDS];
+    stats->band_capacity = ctx->band_capacity;
+    stats->band_retries = retries;
+    stats->tile_rows = p.max_rows;
+    stats->k_padded = p.max_kc * bfj_kchunk(p.dtype);
+    stats->dtype = p.dtype;
+    for (auto &s : sc) stats->max_band_halfwidth = std::max(stats->max_band_halfwidth, bits_to_double(s.halfband_bits));
+    if (timed) {
+        float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+        HC(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+        HC(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+        HC(hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
+        HC(hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
+        stats->ms_prep = t01;
+        stats->ms_join = t12;
+        stats->ms_refine = t23;
+        stats->ms_total = t03;
+    }
+    return BFJ_OK;
+}
+
+// The counters of the pass just enqueued, after the stream has drained.
+static int read_counters(bfj_ctx *ctx, unsigned long long *ctr)
+{
+    HC(hipMemcpyAsync(ctr, ctx->counters.p, BFJ_CTR_ITEMS * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
     return BFJ_OK;
 }
 
@@ -652,52 +721,25 @@ static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>
     int retries = 0;
     for (;;) {
         RC(enqueue_pass(ctx, p, timed));
-        uint32_t ctr[9] = {}; // [0] band records, [1] candidate chunks, [8] middle-stage leftovers
-        HC(hipMemcpyAsync(ctr, ctx->counters.p, sizeof ctr, hipMemcpyDeviceToHost, ctx->stream));
-        HC(hipStreamSynchronize(ctx->stream));
-        const uint32_t band_count = ctr[0];
+        unsigned long long ctr[BFJ_CTR_ITEMS] = {};
+        RC(read_counters(ctx, ctr));
         bool again = false;
         RC(grow_on_overflow(ctx, ctr, &again));
         if (!again) {
-            std::vector<SetScalars> sc(p.sets.size());
-            HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
-            RC(check_finite(sc));
+            std::vector<SetScalars> sc;
+            RC(collect_pass(ctx, p, ctr, retries, timed, sc, stats));
+            ctx->last.reset(new PassPlan(p));
             if (totals_out) {
                 totals_out->assign(p.sets.size(), {});
                 for (size_t si = 0; si < p.sets.size(); ++si)
                     (*totals_out)[si].assign(sc[si].total, sc[si].total + p.sets[si].n_eps);
             }
-            if (stats) {
-                memset(stats, 0, sizeof *stats);
-                stats->pairs = p.pairs;
-                stats->band_pairs = band_count;
-                stats->fp64_pairs = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? ctr[8] : band_count;
-                stats->band_capacity = ctx->band_capacity;
-                stats->band_retries = retries;
-                stats->tile_rows = p.max_rows;
-                stats->k_padded = p.max_kc * bfj_kchunk(p.dtype);
-                stats->dtype = p.dtype;
-                for (auto &s : sc) {
-                    stats->max_abs_d2_err = std::max(stats->max_abs_d2_err, bits_to_double(s.err_bits));
-                    stats->max_band_halfwidth = std::max(stats->max_band_halfwidth, bits_to_double(s.halfband_bits));
-                }
-                if (timed) {
-                    float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
-                    HC(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
-                    HC(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
-                    HC(hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
-                    HC(hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
-                    stats->ms_prep = t01;
-                    stats->ms_join = t12;
-                    stats->ms_refine = t23;
-                    stats->ms_total = t03;
-                }
-            }
             return BFJ_OK;
         }
         if (++retries > 4)
-            return fail(BFJ_ERR_INTERNAL, "candidate stream / refine list keep overflowing (%u chunks, %u records)",
-                        ctr[1], ctr[0]);
+            return fail(BFJ_ERR_INTERNAL, "candidate stream / refine list keep overflowing (%llu chunks, %llu records)",
+                        ctr[BFJ_CTR_CHUNKS], ctr[BFJ_CTR_BAND]);
+        RC(stage_plan(ctx, p)); // the grown buffers may have moved
     }
 }
 
@@ -771,7 +813,7 @@ int bfj_destroy(bfj_ctx *c)
                       &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
                       &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux,
                       &c->perm, &c->label, &c->ohist, &c->oseg, &c->pivots, &c->pivot_norm, &c->mid_rows,
-                      &c->mid_q, &c->mid_band};
+                      &c->mid_q, &c->mid_band, &c->errbuf};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -812,6 +854,13 @@ static int join_one(bfj_ctx *ctx, const double *d_x, int64_t n, int32_t d, int64
     return BFJ_OK;
 }
 
+// Bytes of n rows of d doubles at a stride of ld: the last row ends after its d-th
+// value, which may be the end of the caller's buffer (a column slice of a wider array).
+static size_t rows_bytes(int64_t n, int64_t d, int64_t ld)
+{
+    return n > 0 ? ((size_t)(n - 1) * (size_t)ld + (size_t)d) * 8 : 0;
+}
+
 static int common_checks(bfj_ctx *ctx, const double *data, int64_t n, const double *eps, int32_t n_eps)
 {
     if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
@@ -838,9 +887,10 @@ int bfj_self_join(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, int64_
     RC(common_checks(ctx, data, n, eps, n_eps));
     if (d < 1 || ld < d) return fail(BFJ_ERR_INVALID, "need 1 <= d <= ld");
     if (row_begin < 0 || row_end > n || row_begin > row_end) return fail(BFJ_ERR_INVALID, "bad row range");
-    const size_t xbytes = (size_t)std::max<int64_t>(n, 1) * (size_t)ld * 8;
-    RC(ensure(ctx->xcopy, xbytes));
-    if (n > 0) HC(hipMemcpyAsync(ctx->xcopy.p, data, (size_t)n * ld * 8, hipMemcpyHostToDevice, ctx->stream));
+    ctx->last.reset();
+    const size_t xbytes = rows_bytes(n, d, ld);
+    RC(ensure(ctx->xcopy, std::max<size_t>(xbytes, 8)));
+    if (n > 0) HC(hipMemcpyAsync(ctx->xcopy.p, data, xbytes, hipMemcpyHostToDevice, ctx->stream));
     const size_t obytes = (size_t)std::max<int64_t>(row_end - row_begin, 1) * n_eps * 8;
     RC(ensure(ctx->outbuf, obytes));
     RC(join_one(ctx, (const double *)ctx->xcopy.p, n, d, ld, eps, n_eps, dtype, row_begin, row_end,
@@ -858,9 +908,10 @@ int bfj_self_join_pairs(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, 
     *n_pairs = 0;
     if (capacity > 0 && !pairs) return fail(BFJ_ERR_INVALID, "pairs is NULL");
     if (d < 1 || ld < d) return fail(BFJ_ERR_INVALID, "need 1 <= d <= ld");
+    ctx->last.reset();
     if (n > 0) {
-        RC(ensure(ctx->xcopy, (size_t)n * (size_t)ld * 8));
-        HC(hipMemcpyAsync(ctx->xcopy.p, data, (size_t)n * ld * 8, hipMemcpyHostToDevice, ctx->stream));
+        RC(ensure(ctx->xcopy, rows_bytes(n, d, ld)));
+        HC(hipMemcpyAsync(ctx->xcopy.p, data, rows_bytes(n, d, ld), hipMemcpyHostToDevice, ctx->stream));
     }
     RC(ensure(ctx->pairs, (size_t)std::max<uint64_t>(capacity, 1) * 8));
     ctx->pair_capacity = capacity;
@@ -876,7 +927,7 @@ int bfj_self_join_pairs(bfj_ctx *ctx, const double *data, int64_t n, int32_t d, 
     }
     unsigned long long found = 0;
     SetScalars sc;
-    HC(hipMemcpy(&found, (uint32_t *)ctx->counters.p + 4, 8, hipMemcpyDeviceToHost));
+    HC(hipMemcpy(&found, (unsigned long long *)ctx->counters.p + BFJ_CTR_PAIRS, 8, hipMemcpyDeviceToHost));
     HC(hipMemcpy(&sc, ctx->scal.p, sizeof sc, hipMemcpyDeviceToHost));
     const bool all = (sc.all_mask & 1ull) != 0; // eps beyond every distance: the join left them to us
     if (all) found = n > 1 ? (unsigned long long)n * (unsigned long long)(n - 1) / 2 : 0;
@@ -917,7 +968,7 @@ static int batch_hostsets(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets,
         hs.push_back(h);
         xoff.push_back(xb);
         ooff.push_back(ob);
-        if (s.n > 0 && s.ld > 0) xb += (size_t)s.n * (size_t)s.ld * 8;
+        if (s.n > 0 && s.ld >= s.d && s.d >= 1) xb += (rows_bytes(s.n, s.d, s.ld) + 15) / 16 * 16;
         if (s.n > 0) ob += (size_t)s.n * s.n_eps * 8;
     }
     (void)ctx;
@@ -940,9 +991,10 @@ int bfj_reference_mode_join(bfj_ctx *ctx, const double *data, int64_t n, int32_t
                     search_mode, compute_prec, accum_prec);
     *total = 0;
     if (n == 0) return BFJ_OK;
+    ctx->last.reset(); // xcopy is about to change under the last pass's rows
     const int dpad = (d + 15) / 16 * 16;
-    RC(ensure(ctx->xcopy, (size_t)n * ld * 8));
-    HC(hipMemcpyAsync(ctx->xcopy.p, data, (size_t)n * ld * 8, hipMemcpyHostToDevice, ctx->stream));
+    RC(ensure(ctx->xcopy, rows_bytes(n, d, ld)));
+    HC(hipMemcpyAsync(ctx->xcopy.p, data, rows_bytes(n, d, ld), hipMemcpyHostToDevice, ctx->stream));
     RC(ensure(ctx->ref_x, (size_t)n * dpad * (compute_prec / 8)));
     // Hit counter first, then the FP32 chunk norms of mode 4.
     RC(ensure(ctx->ref_aux, 256 + (size_t)n * (dpad / 16) * sizeof(float)));
@@ -968,11 +1020,13 @@ int bfj_self_join_batch(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, i
     std::vector<HostSet> hs;
     std::vector<size_t> xoff, ooff;
     RC(batch_hostsets(ctx, sets, n_sets, device_resident, hs, xoff, ooff));
+    ctx->last.reset();
     if (!device_resident) {
         // Copy every dataset into one device buffer; counts come back through another.
         size_t xb = 0, ob = 0;
         for (int i = 0; i < n_sets; ++i) {
-            if (sets[i].n > 0 && sets[i].ld > 0) xb = xoff[i] + (size_t)sets[i].n * sets[i].ld * 8;
+            if (sets[i].n > 0 && sets[i].ld >= sets[i].d && sets[i].d >= 1)
+                xb = xoff[i] + rows_bytes(sets[i].n, sets[i].d, sets[i].ld);
             if (sets[i].n > 0) ob = ooff[i] + (size_t)sets[i].n * sets[i].n_eps * 8;
         }
         RC(ensure(ctx->xcopy, std::max<size_t>(xb, 8)));
@@ -980,8 +1034,8 @@ int bfj_self_join_batch(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, i
         for (int i = 0; i < n_sets; ++i) {
             if (sets[i].n > 0 && !sets[i].data) return fail(BFJ_ERR_INVALID, "dataset %d: data is NULL", i);
             if (sets[i].n > 0 && sets[i].ld >= sets[i].d && sets[i].d >= 1)
-                HC(hipMemcpyAsync((char *)ctx->xcopy.p + xoff[i], sets[i].data, (size_t)sets[i].n * sets[i].ld * 8,
-                                  hipMemcpyHostToDevice, ctx->stream));
+                HC(hipMemcpyAsync((char *)ctx->xcopy.p + xoff[i], sets[i].data,
+                                  rows_bytes(sets[i].n, sets[i].d, sets[i].ld), hipMemcpyHostToDevice, ctx->stream));
             hs[i].x = (const double *)((char *)ctx->xcopy.p + xoff[i]);
             hs[i].out = sets[i].counts ? (int64_t *)((char *)ctx->outbuf.p + ooff[i]) : nullptr;
         }
@@ -1009,6 +1063,8 @@ struct bfj_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t captured_gen = 0; // buffer generation the graph was captured with
+    uint32_t captured_chunks = 0; // capacities baked into the captured kernel arguments
+    uint64_t captured_band = 0;
 };
 
 static int capture(bfj_plan *pl)
@@ -1030,6 +1086,8 @@ static int capture(bfj_plan *pl)
     pl->graph = g;
     HC(hipGraphInstantiate(&pl->exec, g, nullptr, nullptr, 0));
     pl->captured_gen = g_buf_gen.load();
+    pl->captured_chunks = ctx->chunk_capacity;
+    pl->captured_band = ctx->band_capacity;
     return BFJ_OK;
 }
 
@@ -1081,46 +1139,23 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
     HC(hipSetDevice(ctx->device));
     // The context's buffers are shared with eager calls: restage if they moved.
     RC(stage_plan(ctx, pl->pass));
-    if (pl->captured_gen != g_buf_gen.load()) RC(capture(pl));
+    if (pl->captured_gen != g_buf_gen.load() || pl->captured_chunks != ctx->chunk_capacity ||
+        pl->captured_band != ctx->band_capacity)
+        RC(capture(pl));
     for (int attempt = 0; attempt < 5; ++attempt) {
+        ctx->last.reset();
         HC(hipGraphLaunch(pl->exec, ctx->stream));
-        uint32_t ctr[9] = {}; // [0] band records, [1] candidate chunks, [8] middle-stage leftovers
-        HC(hipMemcpyAsync(ctr, ctx->counters.p, sizeof ctr, hipMemcpyDeviceToHost, ctx->stream));
-        HC(hipStreamSynchronize(ctx->stream));
-        const uint32_t band_count = ctr[0];
+        unsigned long long ctr[BFJ_CTR_ITEMS] = {};
+        RC(read_counters(ctx, ctr));
         bool again = false;
         RC(grow_on_overflow(ctx, ctr, &again));
         if (!again) {
-            std::vector<SetScalars> sc(pl->pass.sets.size());
-            HC(hipMemcpy(sc.data(), ctx->scal.p, sc.size() * sizeof(SetScalars), hipMemcpyDeviceToHost));
-            RC(check_finite(sc));
+            std::vector<SetScalars> sc;
+            RC(collect_pass(ctx, pl->pass, ctr, attempt, true, sc, stats));
+            ctx->last.reset(new PassPlan(pl->pass));
             for (size_t i = 0; i < pl->sets.size(); ++i)
                 if (pl->sets[i].totals)
                     for (int e = 0; e < pl->sets[i].n_eps; ++e) pl->sets[i].totals[e] = sc[i].total[e];
-            if (stats) {
-                memset(stats, 0, sizeof *stats);
-                stats->pairs = pl->pass.pairs;
-                stats->band_pairs = band_count;
-                stats->fp64_pairs = (pl->pass.dtype != BFJ_FP64 && ctx->mid_stage) ? ctr[8] : band_count;
-                stats->band_capacity = ctx->band_capacity;
-                stats->band_retries = attempt;
-                stats->tile_rows = pl->pass.max_rows;
-                stats->k_padded = pl->pass.max_kc * bfj_kchunk(pl->pass.dtype);
-                stats->dtype = pl->pass.dtype;
-                for (auto &s : sc) {
-                    stats->max_abs_d2_err = std::max(stats->max_abs_d2_err, bits_to_double(s.err_bits));
-                    stats->max_band_halfwidth = std::max(stats->max_band_halfwidth, bits_to_double(s.halfband_bits));
-                }
-                float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
-                HC(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
-                HC(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
-                HC(hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
-                HC(hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
-                stats->ms_prep = t01;
-                stats->ms_join = t12;
-                stats->ms_refine = t23;
-                stats->ms_total = t03;
-            }
             return BFJ_OK;
         }
         RC(stage_plan(ctx, pl->pass));
@@ -1251,8 +1286,105 @@ int bfj_autotune(bfj_ctx *ctx, int32_t dtype, int32_t d, int64_t n, int32_t *til
         }
     }
     (void)hipFree(x);
+    ctx->last.reset(); // its rows are gone
     ctx->tile_rows[dtype][kc] = best;
     if (tile_rows_out) *tile_rows_out = best;
+    return BFJ_OK;
+}
+
+int bfj_set_capacity(bfj_ctx *ctx, uint64_t candidate_chunks, uint64_t band_records)
+{
+    if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
+    if (candidate_chunks > BFJ_MAX_CHUNKS) return fail(BFJ_ERR_INVALID, "candidate_chunks > %llu", BFJ_MAX_CHUNKS);
+    if (band_records > BFJ_MAX_BAND) return fail(BFJ_ERR_INVALID, "band_records > %llu", BFJ_MAX_BAND);
+    // The buffers keep their size; the kernels only use the first `capacity` entries.
+    if (candidate_chunks) ctx->chunk_capacity = (uint32_t)candidate_chunks;
+    if (band_records) ctx->band_capacity = band_records;
+    return BFJ_OK;
+}
+
+// Seeded 16 x 16 tiles over the datasets of a pass, about `want` of them, spread by
+// each dataset's share of the pairs: symmetric datasets over blocks rb <= cb,
+// row-range passes over the query blocks against all blocks.
+static std::vector<ErrTile> sample_tiles(const PassPlan &p, int64_t want, uint64_t seed)
+{
+    std::vector<ErrTile> out;
+    if (want <= 0) return out;
+    auto next = [&seed]() {
+        uint64_t z = (seed += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    double total = 0.0;
+    for (const SetDesc &s : p.sets) total += (double)(s.row_end - s.row_begin) * (double)s.n;
+    if (total <= 0.0) return out;
+    for (size_t si = 0; si < p.sets.size(); ++si) {
+        const SetDesc &s = p.sets[si];
+        const double w = (double)(s.row_end - s.row_begin) * (double)s.n;
+        if (w <= 0.0) continue;
+        const int64_t k = (int64_t)std::ceil((double)want * w / total);
+        const int64_t nb = (s.n + 15) / 16;
+        const int64_t qb0 = s.row_begin / 16, qb1 = (s.row_end + 15) / 16;
+        for (int64_t t = 0; t < k; ++t) {
+            ErrTile e;
+            e.set = (int32_t)si;
+            e.pad = 0;
+            if (s.sym) {
+                int64_t a = (int64_t)(next() % (uint64_t)nb), b = (int64_t)(next() % (uint64_t)nb);
+                e.rb = (int32_t)std::min(a, b);
+                e.cb = (int32_t)std::max(a, b);
+            } else {
+                e.rb = (int32_t)(qb0 + (int64_t)(next() % (uint64_t)(qb1 - qb0)));
+                e.cb = (int32_t)(next() % (uint64_t)nb);
+            }
+            out.push_back(e);
+        }
+    }
+    return out;
+}
+
+int bfj_pass_errors(bfj_ctx *ctx, uint64_t max_candidates, int64_t sample_tiles_n, uint64_t seed, bfj_errors *out)
+{
+    if (!ctx || !out) return fail(BFJ_ERR_INVALID, "ctx or out is NULL");
+    memset(out, 0, sizeof *out);
+    if (sample_tiles_n < 0) return fail(BFJ_ERR_INVALID, "sample_tiles < 0");
+    if (!ctx->last) return fail(BFJ_ERR_INVALID, "no finished pass on this context to measure");
+    HC(hipSetDevice(ctx->device));
+    const PassPlan &p = *ctx->last;
+    const PassArgs a = pass_args(ctx, p);
+    unsigned long long ctr[BFJ_CTR_ITEMS] = {};
+    RC(read_counters(ctx, ctr));
+    const uint32_t n_chunks =
+        (uint32_t)std::min<unsigned long long>(ctr[BFJ_CTR_CHUNKS], (unsigned long long)ctx->chunk_capacity);
+    const unsigned long long total = ctr[BFJ_CTR_CANDS];
+    uint32_t stride = 1;
+    if (max_candidates && total > max_candidates)
+        stride = (uint32_t)std::min<unsigned long long>((total + max_candidates - 1) / max_candidates, 0xffffffffull);
+    const std::vector<ErrTile> tiles = sample_tiles(p, sample_tiles_n, seed);
+    if (tiles.size() > (size_t)0x7fffffff) return fail(BFJ_ERR_INVALID, "too many sample tiles");
+    const size_t tile_off = 256; // ErrOut first
+    RC(ensure(ctx->errbuf, tile_off + std::max<size_t>(tiles.size(), 1) * sizeof(ErrTile)));
+    ErrOut *d_out = (ErrOut *)ctx->errbuf.p;
+    ErrTile *d_tiles = (ErrTile *)((char *)ctx->errbuf.p + tile_off);
+    bfj_launch_zero(d_out, sizeof(ErrOut), ctx->stream);
+    if (!tiles.empty())
+        HC(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(ErrTile), hipMemcpyHostToDevice, ctx->stream));
+    bfj_launch_errcand(a, d_out, n_chunks, stride, ctx->stream);
+    bfj_launch_errtile(a, d_tiles, (uint32_t)tiles.size(), d_out, ctx->stream);
+    HC(hipGetLastError());
+    ErrOut h;
+    HC(hipMemcpyAsync(&h, d_out, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    out->cand_total = total;
+    out->cand_pairs = h.cand_pairs;
+    out->tile_pairs = h.tile_pairs;
+    out->violations = h.violations;
+    out->cand_max_abs_err = h.cand_pairs ? bits_to_double(h.cand_max_bits) : 0.0;
+    out->tile_max_abs_err = h.tile_pairs ? bits_to_double(h.tile_max_bits) : 0.0;
+    const bool low = p.dtype != BFJ_FP64;
+    out->max_acc_ratio = low && h.tile_pairs ? bits_to_double(h.acc_ratio_bits) : -1.0;
+    out->max_quant_ratio = low && h.tile_pairs ? bits_to_double(h.quant_ratio_bits) : -1.0;
     return BFJ_OK;
 }
 

@@ -64,7 +64,6 @@ struct SetScalars {
     unsigned long long maxabs_bits; // max |x - mu| (non-negative double as bits)
     unsigned long long qmax_bits;   // max quantisation error radius (kernel space)
     unsigned long long nmax_bits;   // max ||xhat||^2 (kernel space)
-    unsigned long long err_bits;    // max |d2_approx - d2_fp64| over refined pairs (input space)
     unsigned long long halfband_bits; // max (H - L) / 2 (input space)
     unsigned long long nonfinite;     // non-zero if some row holds NaN or inf
     unsigned long long all_mask;      // eps whose T exceeds every possible distance: all pairs hit
@@ -116,9 +115,20 @@ struct JoinArgs {
     const uint8_t *scales;   // FP8 per-row E8M0 scale bytes (NULL otherwise)
     Candidate *cand;         // candidate stream
     uint32_t *chunk_fill;    // records written into each chunk
-    uint32_t *chunk_count;   // chunks handed out (may exceed chunk_capacity: overflow)
+    unsigned long long *chunk_count; // chunks handed out (may exceed chunk_capacity: overflow)
     uint32_t chunk_capacity;
 };
+
+// Counters of a pass: one buffer zeroed at the start of every pass, 64-bit slots.
+// Every count that can grow with the number of pairs is 64-bit, so none can wrap
+// and hide an overflow from the host (which compares them with the capacities).
+#define BFJ_CTR_BAND 0   // band-list records reserved by k_classify
+#define BFJ_CTR_CHUNKS 1 // candidate chunks handed out by the join
+#define BFJ_CTR_PAIRS 2  // pairs found in pair-list mode
+#define BFJ_CTR_MID 3    // records the middle stage leaves for the FP64 refine
+#define BFJ_CTR_CANDS 4  // candidate records the join wrote (statistics)
+#define BFJ_CTR_ITEMS 8  // from this slot on: 32-bit work-queue heads, one per join launch
+#define BFJ_CTR_SLOTS 32 // 256 bytes
 
 // Host-side launchers (one translation unit per dtype; see join_*.hip).
 struct JoinLaunch {
@@ -160,8 +170,8 @@ struct PassArgs {
     uint32_t *cnt;
     int64_t cnt_len;
     BandEntry *band;
-    uint32_t *band_count;
-    uint32_t band_capacity;
+    unsigned long long *band_count;
+    uint64_t band_capacity;
     // Low-precision passes: a middle stage between classify and the FP64 refine
     // (refine.hip) on higher-precision copies of the rows: FP16 for FP8 passes, FP32
     // for FP16/BF16 passes.
@@ -170,11 +180,12 @@ struct PassArgs {
     unsigned char *mid_rows;           // [row][BFJ_MAXDIM] copy of the kernel-space rows, by position
     double *mid_q;                     // [row] radius of that copy
     BandEntry *mid_band;               // records it leaves for the FP64 refine (capacity band_capacity)
-    uint32_t *mid_count;
+    unsigned long long *mid_count;
     const Candidate *cand;
     const uint32_t *chunk_fill;
-    const uint32_t *chunk_count;
+    const unsigned long long *chunk_count;
     uint32_t chunk_capacity;
+    unsigned long long *cand_records;  // k_classify adds the records it read (statistics)
     int32_t emit_pairs;                // pair-list mode: one dataset, eps 0, every row a query
     uint2 *pairs;                      // (i, j), i < j
     unsigned long long *pair_count;    // pairs found (may exceed pair_capacity)
@@ -201,6 +212,16 @@ __device__ inline uint32_t bfj_row_at(const PassArgs &a, const SetDesc &s, int64
 #define BFJ_MAXDIM 256
 #define BFJ_PREP_ROWS 64   // rows per block in row-parallel prep kernels
 
+// OCP FP8 e4m3 byte -> its value (bias 7, subnormals m * 2^-9; the encoder in prep.hip
+// never produces the NaN code 0x7f / 0xff).
+__device__ inline double e4m3_decode(uint8_t b)
+{
+    const double sg = (b & 0x80) ? -1.0 : 1.0;
+    const int ef = (b >> 3) & 15, m = b & 7;
+    if (ef == 0) return sg * ldexp((double)m, -9);
+    return sg * ldexp((double)(8 + m), ef - 10);
+}
+
 void bfj_launch_zero(void *p, size_t bytes, hipStream_t s);
 void bfj_launch_prep(const PassArgs &a, hipStream_t s);
 void bfj_launch_classify(const PassArgs &a, hipStream_t s);
@@ -209,3 +230,14 @@ void bfj_launch_mid(const PassArgs &a, hipStream_t s);
 void bfj_launch_finalize(const PassArgs &a, hipStream_t s);
 void bfj_launch_all_pairs(uint2 *pairs, int64_t n, hipStream_t s);
 void bfj_launch_order(const PassArgs &a, hipStream_t s);
+
+// Accuracy statistics of a finished pass (errstat.hip).
+struct ErrTile {
+    int32_t set, rb, cb, pad; // dataset, row block and candidate block of one sampled 16 x 16 tile
+};
+struct ErrOut { // zeroed before the statistics kernels; maxima as bits of non-negative doubles
+    unsigned long long cand_pairs, tile_pairs, violations;
+    unsigned long long cand_max_bits, tile_max_bits, acc_ratio_bits, quant_ratio_bits;
+};
+void bfj_launch_errcand(const PassArgs &a, ErrOut *out, uint32_t n_chunks, uint32_t stride, hipStream_t s);
+void bfj_launch_errtile(const PassArgs &a, const ErrTile *tiles, uint32_t n_tiles, ErrOut *out, hipStream_t s);

@@ -124,6 +124,16 @@ int GPUJoinMainBruteForceReference(unsigned int searchMode, unsigned int device,
     return BFJ_OK;
 }
 
+int GPUJoinPassErrors(unsigned int device, uint64_t maxCandidates, int64_t sampleTiles, uint64_t seed,
+                      bfj_errors *errors)
+{
+    bfj_ctx *ctx = context_for(device);
+    if (!ctx) return BFJ_ERR_DEVICE;
+    const int rc = bfj_pass_errors(ctx, maxCandidates, sampleTiles, seed, errors);
+    if (rc != BFJ_OK) std::fprintf(stderr, "[GPU] ~ Error: %s\n", bfj_last_error());
+    return rc;
+}
+
 void GPUJoinMainBruteForce(unsigned int searchMode, unsigned int device, INPUT_DATA_TYPE *dataset,
                            unsigned int *nbQueryPoints, ACCUM_TYPE *epsilon, uint64_t *totalNeighbors)
 {

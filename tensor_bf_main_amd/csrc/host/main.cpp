@@ -2,7 +2,7 @@
 //
 //   bfjoin_main <file> <eps> <searchMode> [device] [--dim D] [--compute-dim C]
 //               [--binary] [--counts out.txt] [--pairs out.txt] [--log file] [--parse-only]
-//               [--reference-arithmetic [--compute-prec P] [--accum-prec P]]
+//               [--eps-double] [--reference-arithmetic [--compute-prec P] [--accum-prec P]]
 //
 // Same positional arguments, console lines and CSV result log as the reference;
 // the options give the row length at run time (the reference fixes it at compile
@@ -41,12 +41,17 @@ int main(int argc, char *argv[])
     if (argc < 4) {
         std::cerr << "usage: " << argv[0]
                   << " <file> <eps> <searchMode> [device] [--dim D] [--compute-dim C] [--binary]"
-                     " [--counts out.txt] [--pairs out.txt] [--log file]"
+                     " [--counts out.txt] [--pairs out.txt] [--log file] [--eps-double]"
                      " [--reference-arithmetic [--compute-prec P] [--accum-prec P]]\n";
         return 1;
     }
     const char *filename = argv[FILENAME_ARG];
-    const double epsilon = std::atof(argv[EPSILON_ARG]);
+    // main.cu:20 parses eps into ACCUM_TYPE (float unless ACCUM_PREC is 64): the same
+    // argument string gives the same eps as the reference, and the same eps as the
+    // drop-in GPUJoinMainBruteForce, which takes an ACCUM_TYPE*.  --eps-double keeps
+    // the full double instead.
+    const double eps_arg = std::atof(argv[EPSILON_ARG]);
+    double epsilon = (double)(ACCUM_TYPE)eps_arg;
     const unsigned int searchMode = (unsigned)std::atoi(argv[SEARCHMODE_ARG]);
     unsigned int device = 0;
     int dim = INPUT_DATA_DIM, computeDim = -1;
@@ -64,6 +69,7 @@ int main(int argc, char *argv[])
         else if (!std::strcmp(argv[a], "--pairs") && a + 1 < argc) pairs_path = argv[++a];
         else if (!std::strcmp(argv[a], "--log") && a + 1 < argc) log_path = argv[++a];
         else if (!std::strcmp(argv[a], "--reference-arithmetic")) ref_arith = true;
+        else if (!std::strcmp(argv[a], "--eps-double")) epsilon = eps_arg;
         else if (!std::strcmp(argv[a], "--compute-prec") && a + 1 < argc) computePrec = std::atoi(argv[++a]);
         else if (!std::strcmp(argv[a], "--accum-prec") && a + 1 < argc) accumPrec = std::atoi(argv[++a]);
         else if (pos++ == 0) device = (unsigned)std::atoi(argv[a]); // DEVICE_ARG
@@ -149,6 +155,15 @@ int main(int argc, char *argv[])
                                                       counts.empty() ? nullptr : counts.data(), &stats);
     const double timeJoin = now_s() - tJoin0;
     if (rc != BFJ_OK) return 1;
+    // Accuracy of the pass's approximate distances (after the timed region): the max
+    // |d2_approx - d2_fp64| over its candidate pairs, for the log.
+    bfj_errors errs;
+    std::memset(&errs, 0, sizeof errs);
+    if (!ref_mode && GPUJoinPassErrors(device, 1u << 22, 256, 12345, &errs) != BFJ_OK) return 1;
+    if (errs.violations) {
+        std::cerr << "[Main] ~ Error: " << errs.violations << " pairs outside the error bound\n";
+        return 1;
+    }
     std::cout << "[Main | Result] ~ Time to join: " << timeJoin << '\n';
     std::cout << "[Main | Result] ~ Total result set size: " << totalResult << '\n';
 
@@ -182,7 +197,7 @@ int main(int argc, char *argv[])
     out << filename << ", " << epsilon << ", " << searchMode << ", " << timeJoin << ", " << totalResult << ", " << dim
         << ", " << computeDim << ", " << BLOCKSIZE << ", " << WARP_PER_BLOCK << ", " << computePrec << ", "
         << accumPrec << ", " << (ref_mode ? "reference" : dtype_name(stats.dtype)) << ", " << (timeJoin > 0 ? pairs / timeJoin : 0.0) << ", "
-        << stats.band_pairs << ", " << stats.max_abs_d2_err << ", " << stats.tile_rows << ", " << stats.ms_total
+        << stats.band_pairs << ", " << errs.cand_max_abs_err << ", " << stats.tile_rows << ", " << stats.ms_total
         << std::endl;
     return 0;
 }

@@ -235,14 +235,6 @@ __device__ inline uint8_t e4m3_encode(double v)
     return sgn | (uint8_t)(((E + 7) << 3) | ((int)m - 8));
 }
 
-__device__ inline double e4m3_decode(uint8_t b)
-{
-    const double sg = (b & 0x80) ? -1.0 : 1.0;
-    const int ef = (b >> 3) & 15, m = b & 7;
-    if (ef == 0) return sg * ldexp((double)m, -9);
-    return sg * ldexp((double)(8 + m), ef - 10);
-}
-
 // One wavefront per row; 4 wavefronts x 16 rows per block.
 __global__ __launch_bounds__(BFJ_BLOCK) void k_quantize(PassArgs a)
 {

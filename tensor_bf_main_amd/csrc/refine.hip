@@ -318,11 +318,14 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
     CountTable &tab = tabs[threadIdx.x >> 6];
     const bool use_tab = a.cnt_len < (int64_t)BFJ_CTAB_EMPTY;
     if (use_tab) ctab_clear(tab, lane);
-    const uint32_t n_chunks = min(*a.chunk_count, a.chunk_capacity);
+    const unsigned long long chunks_used = *a.chunk_count;
+    const uint32_t n_chunks = chunks_used < a.chunk_capacity ? (uint32_t)chunks_used : a.chunk_capacity;
+    unsigned long long records_read = 0;
     const uint32_t wave_id = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
     const uint32_t n_waves = gridDim.x * (BFJ_BLOCK / BFJ_WAVE);
     for (uint32_t ch = wave_id; ch < n_chunks; ch += n_waves) {
         const uint32_t fill = min(a.chunk_fill[ch], (uint32_t)BFJ_CHUNK);
+        records_read += fill;
         const Candidate *recs = a.cand + (size_t)ch * BFJ_CHUNK;
         const uint32_t n_groups = (fill + BFJ_WAVE - 1) / BFJ_WAVE; // at most 16
         // Pass 1: count the certain-in pairs, note which records go to the refine list.
@@ -347,15 +350,15 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
         if (!n_band) continue;
         // One atomic per chunk reserves its refine-list records (a counter that every
         // wavefront hits: per chunk rather than per 64 records keeps it uncontended).
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.band_count, n_band);
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(a.band_count, (unsigned long long)n_band);
         base = __shfl(base, 0);
         // Pass 2: decide the band groups again and write their records.
         auto band_body = [&](const Decided &d, const auto &, uint32_t) {
             const bool band = (d.band_i | d.band_j) != 0u;
             const unsigned long long bb = __ballot(band);
             if (band) {
-                const uint32_t idx = base + (uint32_t)__popcll(bb & below);
+                const unsigned long long idx = base + (unsigned long long)__popcll(bb & below);
                 if (idx < a.band_capacity) {
                     BandEntry be;
                     be.i = d.i;
@@ -365,7 +368,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
                     a.band[idx] = be;
                 }
             }
-            base += (uint32_t)__popcll(bb);
+            base += (unsigned long long)__popcll(bb);
         };
         for (uint32_t g0 = 0; g0 < n_groups; g0 += G) {
             const uint32_t m = (band_groups >> g0) & ((1u << G) - 1u);
@@ -373,6 +376,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
         }
     }
     if (use_tab) ctab_flush(a, tab, lane);
+    if (lane == 0 && records_read) atomicAdd(a.cand_records, records_read);
 }
 
 void bfj_launch_classify(const PassArgs &a, hipStream_t st)
@@ -414,12 +418,11 @@ __device__ inline double fma_chunk(const double (&u)[RK], const double (&v)[RK],
 }
 
 // Refine one band record per lane (lanes with live == false take part in the votes
-// and decide nothing): the sequential FMA chain of the pair's two FP64 rows, the
+// and decide nothing): the sequential FMA chain of the pair's two FP64 rows and the
 // bucket of the first eps (ascending) whose threshold holds it, counted for its row
-// and (symmetric record) for the other row, and the approximation error for the
-// statistics.  All lanes take part.
+// and (symmetric record) for the other row.  All lanes take part.
 __device__ inline void refine_one(const PassArgs &a, const BandEntry &be, bool live, int lane, CountTable &tab,
-                                  bool use_tab, int &err_set, double &err_run)
+                                  bool use_tab)
 {
     const int si = (int)(be.meta >> 16);
     const SetDesc &s = a.sets[si];
@@ -466,36 +469,6 @@ __device__ inline void refine_one(const PassArgs &a, const BandEntry &be, bool l
             }
     count_rows(a, tab, use_tab, s, be.i, k, be.j, (be.meta & 0x8000u) ? k : -1, lane);
     if (a.emit_pairs) append_pairs(a, s, k == 0 && be.i != be.j, be.i, be.j, lane);
-    if (live) {
-        const double sigma = a.scal[si].sigma;
-        const double approx = sigma * sigma * (a.he[s.row_off + be.i].x - 2.0 * (double)be.z);
-        const double err = fabs(approx - acc);
-        // Largest approximation error per dataset, kept in a register while the
-        // dataset does not change and flushed by refine_flush (one atomic on a
-        // counter every wavefront shares would serialise the whole kernel).
-        if (si != err_set) {
-            if (err_set >= 0 && err_run >= 0.0)
-                atomicMax(&a.scal[err_set].err_bits, (unsigned long long)__double_as_longlong(err_run));
-            err_set = si;
-            err_run = -1.0;
-        }
-        err_run = fmax(err_run, err);
-    }
-}
-
-// Flush of the running error maxima: one atomic per wavefront for the common case
-// of one dataset.  All lanes take part.
-__device__ inline void refine_flush(const PassArgs &a, int err_set, double err_run, int lane)
-{
-    const int set0 = __shfl(err_set, 0);
-    if (__ballot(err_set != set0) == 0ull) {
-        double mx = err_run;
-        for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
-        if (lane == 0 && set0 >= 0 && mx >= 0.0)
-            atomicMax(&a.scal[set0].err_bits, (unsigned long long)__double_as_longlong(mx));
-    } else if (err_set >= 0 && err_run >= 0.0) {
-        atomicMax(&a.scal[err_set].err_bits, (unsigned long long)__double_as_longlong(err_run));
-    }
 }
 
 // Each wavefront takes runs of BFJ_REFINE_RUN consecutive records (classify writes a
@@ -507,7 +480,7 @@ __device__ inline void refine_flush(const PassArgs &a, int err_set, double err_r
 __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t cnt = *a.band_count;
+    const unsigned long long cnt = *a.band_count;
     const uint64_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
     const uint64_t wave = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
     const uint64_t step = (uint64_t)gridDim.x * (BFJ_BLOCK / BFJ_WAVE) * BFJ_REFINE_RUN;
@@ -515,8 +488,6 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
     CountTable &tab = tabs[threadIdx.x >> 6];
     const bool use_tab = a.cnt_len < (int64_t)BFJ_CTAB_EMPTY;
     if (use_tab) ctab_clear(tab, lane);
-    int err_set = -1;
-    double err_run = -1.0;
     // base is uniform over the wavefront, so every lane takes part in the votes.
     for (uint64_t run = wave * BFJ_REFINE_RUN; run < m; run += step) {
         const uint64_t end = run + BFJ_REFINE_RUN < m ? run + BFJ_REFINE_RUN : m;
@@ -528,11 +499,10 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
             be.z = 0.f;
             be.meta = 0;
             if (live) be = a.band[t];
-            refine_one(a, be, live, lane, tab, use_tab, err_set, err_run);
+            refine_one(a, be, live, lane, tab, use_tab);
         }
     }
     if (use_tab) ctab_flush(a, tab, lane);
-    refine_flush(a, err_set, err_run, lane);
 }
 
 // Pair-list mode when eps covers every pair: all (i, j), i < j, row i's pairs at
@@ -643,7 +613,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull;
-    const uint32_t cnt = *a.band_count;
+    const unsigned long long cnt = *a.band_count;
     const uint64_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
     const uint64_t wave = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
     const uint64_t step = (uint64_t)gridDim.x * (BFJ_BLOCK / BFJ_WAVE) * BFJ_REFINE_RUN;
@@ -718,11 +688,11 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
             if (a.emit_pairs) append_pairs(a, s, k == 0 && be.i != be.j, be.i, be.j, lane);
             const unsigned long long fb = __ballot(fwd);
             if (fb) {
-                uint32_t b0 = 0;
-                if (lane == 0) b0 = atomicAdd(a.mid_count, (uint32_t)__popcll(fb));
+                unsigned long long b0 = 0;
+                if (lane == 0) b0 = atomicAdd(a.mid_count, (unsigned long long)__popcll(fb));
                 b0 = __shfl(b0, 0);
                 if (fwd) {
-                    const uint32_t idx = b0 + (uint32_t)__popcll(fb & below);
+                    const unsigned long long idx = b0 + (unsigned long long)__popcll(fb & below);
                     if (idx < a.band_capacity) a.mid_band[idx] = be;
                 }
             }

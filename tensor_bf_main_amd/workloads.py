@@ -93,17 +93,25 @@ def config(k: int, n: int | None = None, d: int | None = None):
 
 def batch(n_sets: int = 512, seed: int = SEED_BASE + 5, n_min: int = 1000, n_max: int = 65536,
           dims=(16, 64, 128, 200), quantiles=(0.001, 0.005, 0.01, 0.02)):
-    """Workload 5: list of {"data", "eps"} dicts."""
+    """Workload 5: list of {"data", "eps"} dicts.  Each dataset's shape, kind and seed
+    are drawn in order from one generator; the datasets themselves come from their own
+    seeds, so they are generated in parallel (threads: numpy's generators release the
+    GIL) with the same values as one after the other."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+
     rng = np.random.default_rng(seed)
-    sets = []
-    for s in range(n_sets):
+    specs = []
+    for _ in range(n_sets):
         n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
         d = int(rng.choice(dims))
         sub = int(rng.integers(0, 2**31))
-        if rng.random() < 0.5:
-            x = uniform(n, d, sub)
-        else:
-            x = gmm(n, d, 8, 0.05, sub)
-        eps = eps_quantile(x, list(quantiles), 500, sub + 1)
-        sets.append({"data": x, "eps": eps})
-    return sets
+        specs.append((n, d, sub, rng.random() < 0.5))
+
+    def make(spec):
+        n, d, sub, is_uniform = spec
+        x = uniform(n, d, sub) if is_uniform else gmm(n, d, 8, 0.05, sub)
+        return {"data": x, "eps": eps_quantile(x, list(quantiles), 500, sub + 1)}
+
+    with ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1))) as ex:
+        return list(ex.map(make, specs))

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 14
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.bfj_abi_version() == bfj.ABI_VERSION == 2
+    assert lib.bfj_abi_version() == bfj.ABI_VERSION == 3
 
 
 def test_cli_exports_reference_entry_points():
@@ -77,6 +77,25 @@ def test_validation_without_device():
         with pytest.raises(bfj.JoinError) as ei:
             bfj.Context(0)
         assert ei.value.code == 2
+
+
+def test_host_rows_keeps_views_and_copies_the_rest():
+    """The host-buffer entry points take rows at a stride ld >= d from the array's data
+    pointer: column slices and every-k-th-row views go through as they are (no copy);
+    reversed, broadcast or column-strided arrays are copied first, never passed with a
+    stride the library would read outside the buffer with."""
+    wide = np.arange(70.0 * 50).reshape(70, 50)
+    for view, ld in ((wide[:, 10:50], 50), (wide[::3, :17], 150), (wide, 50)):
+        x, n, d, ld_got = bfj.host_rows(view)
+        assert x is view and (n, d) == view.shape and ld_got == ld
+        assert x.ctypes.data + ((n - 1) * ld_got + d) * 8 <= wide.ctypes.data + wide.nbytes
+    for view in (wide[::-1], wide[:, ::2], np.broadcast_to(wide[0], (5, 50))):
+        x, n, d, ld = bfj.host_rows(view)
+        assert x.flags.c_contiguous and ld == d and np.array_equal(x, view)
+    x, n, d, ld = bfj.host_rows(wide[3:4, 5:9])
+    assert (n, d, ld) == (1, 4, 4)
+    with pytest.raises(ValueError):
+        bfj.host_rows(wide.astype(np.float32))
 
 
 def test_tile_options_and_errors():

@@ -18,6 +18,7 @@ from tensor_bf_main_amd import workloads as wl
 pytestmark = pytest.mark.gpu
 DTYPES = ["fp64", "fp16", "bf16", "fp8"]
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+ROOT_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _check(ctx, x, eps, dtype, row_begin=0, row_end=None):
@@ -131,9 +132,54 @@ def test_workload2_reduced(ctx, dtype):
 @pytest.mark.parametrize("dtype", DTYPES)
 def test_workload3_reduced(ctx, dtype):
     x, eps = wl.config(3, n=30000)
-    res = _check(ctx, x, eps, dtype)
-    if dtype == "fp16":
-        assert res.stats["max_band_halfwidth"] > res.stats["max_abs_d2_err"]
+    _check(ctx, x, eps, dtype)
+
+
+def _check_errors(ctx, res, dtype, tiles=2048):
+    """bfj_pass_errors right after a pass: every candidate record and the sampled tiles
+    lie inside the interval the band tests assumed, and the two bound ratios measured
+    on the cast rows are at most 1 (the accumulation bound E_i and the quantisation
+    radius q_i + q_j each hold on their own)."""
+    e = ctx.pass_errors(max_candidates=0, sample_tiles=tiles, seed=7)
+    assert e["violations"] == 0, e
+    assert e["cand_total"] == res.stats["candidates"] and e["cand_pairs"] == e["cand_total"], e
+    assert e["tile_pairs"] > 0
+    if dtype == "fp64":
+        assert e["max_acc_ratio"] == -1.0 and e["max_quant_ratio"] == -1.0
+    else:
+        assert 0.0 < e["max_acc_ratio"] <= 1.0, e
+        assert 0.0 < e["max_quant_ratio"] <= 1.0, e
+    return e
+
+
+@pytest.mark.parametrize("workload,n", [(3, 20000), (4, 8000), (2, 10000)])
+def test_pass_errors_within_the_certified_bounds(ctx, workload, n):
+    """The max squared-distance error of every dtype, measured against the FP64 direct
+    form, on the workloads' own data: FP64 is orders of magnitude closer than FP16/BF16,
+    which are closer than FP8, and every dtype stays inside its certified interval."""
+    x, eps = wl.config(workload, n=n)
+    errs = {}
+    for dtype in DTYPES:
+        res = _check(ctx, x, eps, dtype)
+        errs[dtype] = _check_errors(ctx, res, dtype)
+    tile = {k: v["tile_max_abs_err"] for k, v in errs.items()}
+    assert tile["fp64"] < 1e-6 * tile["fp16"], tile
+    assert tile["fp16"] < tile["fp8"] and tile["bf16"] < tile["fp8"], tile
+    assert tile["fp16"] > 0.0
+
+
+def test_pass_errors_sampling_and_errors(ctx):
+    x, eps = wl.config(3, n=6000)
+    res = ctx.self_join(x, eps, dtype="fp16")
+    full = ctx.pass_errors(max_candidates=0, sample_tiles=0)
+    assert full["tile_pairs"] == 0 and full["cand_pairs"] == res.stats["candidates"]
+    part = ctx.pass_errors(max_candidates=max(1, res.stats["candidates"] // 10), sample_tiles=16)
+    assert 0 < part["cand_pairs"] < full["cand_pairs"] and part["cand_max_abs_err"] <= full["cand_max_abs_err"]
+    # Another call on the context invalidates the measurement.
+    ctx.reference_mode_join(x[:50], 0.1, 1, 64, 64)
+    with pytest.raises(bfj.JoinError) as ei:
+        ctx.pass_errors()
+    assert ei.value.code == 1
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -193,6 +239,95 @@ def test_plan_graph_replays(ctx):
     plan.close()
 
 
+def test_items_of_a_destroyed_plan_are_never_reused(ctx):
+    """A plan's work items live as long as the plan.  After it is destroyed, an eager
+    call with a new shape must upload its own items, whatever address its item list
+    gets (it used to be recognised by address, and a new list at the freed address was
+    taken for the plan's)."""
+    sets = wl.batch(n_sets=3, seed=79, n_min=300, n_max=1500)
+    dev = [{"data": ctx.to_device(s["data"]), "eps": s["eps"],
+            "counts": ctx.empty((len(s["eps"]), s["data"].shape[0]), np.int64)} for s in sets]
+    xa = wl.uniform(2100, 24, 1)
+    xb = wl.gmm(3300, 40, 5, 0.05, 2)
+    ea = wl.eps_quantile(xa, [0.01], 200, 1)
+    eb = wl.eps_quantile(xb, [0.005, 0.02], 200, 2)
+    for rep in range(3):
+        plan = ctx.plan(dev, dtype="fp16")
+        _check(ctx, xa, ea, "fp16")
+        plan.run()
+        for dv, s in zip(dev, sets):
+            np.testing.assert_array_equal(dv["counts"].numpy(), oracle.join(s["data"], s["eps"])[0])
+        plan.close()
+        _check(ctx, xb[: 3300 - 100 * rep], eb, "fp16")
+
+
+def test_column_slice_at_the_end_of_its_buffer(tmp_path):
+    """A column slice of a wider array whose last row ends where the buffer ends, with
+    an unreadable page right after it: the host-buffer entry points copy exactly
+    (n - 1) ld + d values (they used to copy n ld from the slice's start, past the end).
+    Run in a child process, so that a read past the end fails that process only."""
+    import subprocess
+    import sys
+    code = f"""
+import ctypes, mmap, sys
+import numpy as np
+sys.path.insert(0, {ROOT_DIR!r})
+import tensor_bf_main_amd as bfj
+from oracle import oracle
+n, ld, d0, d = 900, 50, 10, 40
+page = mmap.PAGESIZE
+nbytes = n * ld * 8
+total = ((nbytes + page - 1) // page + 1) * page
+m = mmap.mmap(-1, total)
+base = ctypes.addressof(ctypes.c_char.from_buffer(m))
+libc = ctypes.CDLL(None)
+libc.mprotect.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+assert libc.mprotect(base + total - page, page, 0) == 0  # PROT_NONE after the array
+wide = np.frombuffer(m, dtype=np.float64, count=n * ld, offset=total - page - nbytes).reshape(n, ld)
+wide[:] = np.random.default_rng(3).random((n, ld))
+x = wide[:, d0:]
+eps = [0.9, 1.2]
+with bfj.Context(0) as ctx:
+    for dt in ("fp16", "fp64"):
+        res = ctx.self_join(x, eps, dtype=dt)
+        oc, ot = oracle.join(x, eps)
+        assert np.array_equal(res.counts, oc) and np.array_equal(res.totals, ot), dt
+        pairs, tot, _ = ctx.self_join_pairs(np.ascontiguousarray(x), eps[0], dtype=dt)
+        assert tot == int(ot[0])
+print("ok")
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.returncode, r.stderr[-2000:])
+
+
+@pytest.mark.parametrize("dtype", ["fp16", "fp64", "fp8"])
+def test_forced_overflow_grows_and_repeats(ctx, dtype):
+    """Capacities far below what a pass needs: the candidate stream and the band list
+    overflow, the library grows both to what the pass reported and repeats it; counts
+    are exact and band_retries says it happened.  Eager calls and a captured plan."""
+    x, eps = wl.config(3, n=9000)
+    ctx.set_capacity(candidate_chunks=2, band_records=16)
+    res = _check(ctx, x, eps, dtype)
+    assert res.stats["band_retries"] >= 1
+    assert res.stats["candidates"] > 2 * 1024
+    # The grown capacities stay: the next pass runs once.
+    res = _check(ctx, x, eps, dtype)
+    assert res.stats["band_retries"] == 0
+    sets = wl.batch(n_sets=4, seed=81, n_min=1000, n_max=3000)
+    dev = [{"data": ctx.to_device(s["data"]), "eps": s["eps"],
+            "counts": ctx.empty((len(s["eps"]), s["data"].shape[0]), np.int64)} for s in sets]
+    plan = ctx.plan(dev, dtype=dtype)
+    ctx.set_capacity(candidate_chunks=1, band_records=8)
+    st = plan.run()
+    assert st["band_retries"] >= 1
+    for dv, s, tot in zip(dev, sets, plan.totals()):
+        oc, ot = oracle.join(s["data"], s["eps"])
+        np.testing.assert_array_equal(dv["counts"].numpy(), oc)
+        np.testing.assert_array_equal(tot, ot)
+    assert plan.run()["band_retries"] == 0
+    plan.close()
+
+
 def test_device_resident_path(ctx):
     x, eps = wl.config(1, n=5000)
     xd = ctx.to_device(x)
@@ -246,6 +381,84 @@ def test_workload3_full_fp16_vs_fp64(ctx):
     rows = np.random.default_rng(4242).choice(n, 1000, replace=False)
     oc = oracle.join_rows(x, rows, eps)[0]
     np.testing.assert_array_equal(a16[rows], oc)
+
+
+@pytest.mark.slow
+def test_workload2_full_fp64_and_fp16():
+    """n = 100k, d = 128 uniform: the FP64-MFMA and FP16 paths agree on every per-point
+    count, 1000 seeded rows match the oracle, and both stay inside their certified error
+    intervals."""
+    import tensor_bf_main_amd as bfj_
+    x, eps = wl.config(2)
+    n = x.shape[0]
+    with bfj_.Context(0) as c:
+        xd = c.to_device(x)
+        got = {}
+        for dtype in ("fp64", "fp16"):
+            cnt = c.empty((1, n), np.int64)
+            res = c.self_join_device(xd, eps, dtype=dtype, counts=cnt)
+            e = c.pass_errors(max_candidates=1 << 22, sample_tiles=1024)
+            assert e["violations"] == 0, (dtype, e)
+            got[dtype] = (cnt.numpy()[0], res.totals[0])
+        np.testing.assert_array_equal(got["fp16"][0], got["fp64"][0])
+        assert got["fp16"][1] == got["fp64"][1]
+        rows = np.random.default_rng(4343).choice(n, 1000, replace=False)
+        np.testing.assert_array_equal(got["fp64"][0][rows], oracle.join_rows(x, rows, eps)[0])
+        # about 64 neighbours per point, as the workload asks
+        assert 32 < (int(got["fp64"][1]) - n) / n < 128
+
+
+@pytest.mark.slow
+def test_workload4_full_bf16_fp8_vs_fp64():
+    """n = 500k, d = 256 unit-norm rows with 5 % near-duplicates: BF16 and FP8 (per-row
+    scales) each give the FP64-MFMA join's per-point counts exactly; 500 seeded rows
+    match the oracle; no pair leaves its certified interval."""
+    import tensor_bf_main_amd as bfj_
+    x, eps = wl.config(4)
+    n = x.shape[0]
+    with bfj_.Context(0) as c:
+        xd = c.to_device(x)
+        cnt = c.empty((1, n), np.int64)
+        r64 = c.self_join_device(xd, eps, dtype="fp64", counts=cnt)
+        want = cnt.numpy()[0]
+        rows = np.random.default_rng(4444).choice(n, 500, replace=False)
+        np.testing.assert_array_equal(want[rows], oracle.join_rows(x, rows, eps)[0])
+        for dtype in ("bf16", "fp8"):
+            res = c.self_join_device(xd, eps, dtype=dtype, counts=cnt)
+            e = c.pass_errors(max_candidates=1 << 22, sample_tiles=1024)
+            assert e["violations"] == 0, (dtype, e)
+            np.testing.assert_array_equal(cnt.numpy()[0], want, err_msg=dtype)
+            assert res.totals[0] == r64.totals[0]
+            assert res.stats["fp64_pairs"] <= res.stats["band_pairs"]
+
+
+@pytest.mark.slow
+def test_workload5_full_graph_vs_eager():
+    """512 datasets x 4 eps in one captured plan: every replay gives the per-dataset
+    eager counts, and a seeded few whole datasets match the oracle."""
+    import tensor_bf_main_amd as bfj_
+    sets = wl.batch(n_sets=512)
+    with bfj_.Context(0) as c:
+        dev = [{"data": c.to_device(s["data"]), "eps": s["eps"],
+                "counts": c.empty((len(s["eps"]), s["data"].shape[0]), np.int64)} for s in sets]
+        plan = c.plan(dev, dtype="fp16")
+        plan.run()
+        e = c.pass_errors(max_candidates=1 << 22, sample_tiles=2048)
+        assert e["violations"] == 0, e
+        st = plan.run()
+        assert st["pairs"] == sum(s["data"].shape[0] ** 2 for s in sets)
+        graph = [dv["counts"].numpy() for dv in dev]
+        gtot = plan.totals()
+        plan.close()
+        for k, (dv, s) in enumerate(zip(dev, sets)):
+            res = c.self_join_device(dv["data"], s["eps"], dtype="fp16", counts=dv["counts"])
+            np.testing.assert_array_equal(dv["counts"].numpy(), graph[k], err_msg=f"set {k}")
+            np.testing.assert_array_equal(res.totals, gtot[k])
+        small = [k for k, s in enumerate(sets) if s["data"].shape[0] <= 6000]
+        for k in np.random.default_rng(5).choice(small, 6, replace=False):
+            oc, ot = oracle.join(sets[k]["data"], sets[k]["eps"])
+            np.testing.assert_array_equal(graph[k], oc, err_msg=f"set {k}")
+            np.testing.assert_array_equal(gtot[k], ot)
 
 
 @pytest.mark.parametrize("mode,dtype", [(1, "fp64"), (2, "fp64"), (3, "fp16"), (4, "fp16"), (7, "fp64"),

@@ -13,6 +13,7 @@ parity with the reference itself is unpinned.
 """
 import os
 import subprocess
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -123,6 +124,88 @@ def test_integer_lattice_is_exact(mode, cp, ap):
     for eps in (0.0, 1.0, np.sqrt(5.5), 4.5):
         _, tot = oracle.join(x, eps)
         assert oracle.reference_total(x, eps, mode, cp, ap) == int(tot[0]), eps
+
+
+# ---- exact-rational emulation of the same-type combinations ---------------------------
+# When COMPUTE_PREC == ACCUM_PREC nvcc's default contraction (-fmad=true) fuses a product
+# into the sum that consumes it.  numpy has no fused multiply-add, so these three
+# combinations are pinned by an emulation in exact rationals with one correctly rounded
+# step per operation, of the reference's own expressions (kernel_join.cu:184-189 and
+# :222-231), in both the fused and the unfused reading; the data are chosen so that the
+# two readings give different totals, so the check can tell them apart.
+
+def _round(x: Fraction, p: int, emin: int) -> Fraction:
+    """x rounded to nearest even in binary floating point with p-bit significands and
+    minimum exponent emin (binary32: 24, -126; binary64: 53, -1022)."""
+    if x == 0:
+        return Fraction(0)
+    sgn, a = (-1 if x < 0 else 1), abs(x)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    if Fraction(2) ** e > a:
+        e -= 1
+    e = max(e, emin)
+    ulp = Fraction(2) ** (e - p + 1)
+    m = a / ulp
+    q, r = divmod(m.numerator, m.denominator)
+    if 2 * r > m.denominator or (2 * r == m.denominator and q % 2 == 1):
+        q += 1
+    return sgn * q * ulp
+
+
+def _emulate_d2(x, mode, prec, fused):
+    """acc for every ordered pair of rows of x (compute = accumulate = binary32 or 64)."""
+    p, emin = (24, -126) if prec == 32 else (53, -1022)
+    R = lambda v: _round(v, p, emin)  # noqa: E731
+    X = [[R(Fraction(float(v))) for v in row] for row in x]  # convertDataset to COMPUTE_TYPE
+    n, d = len(X), len(X[0])
+    out = np.zeros((n, n))
+    for i in range(n):
+        q = X[i]
+        if mode == 2:
+            pt = [R(-2 * v) for v in q]   # (COMPUTE_TYPE)(-2.0) * queryCoord
+            q2 = [R(v * v) for v in q]     # (ACCUM_TYPE)q * (ACCUM_TYPE)q, its own statement
+        for j in range(n):
+            c = X[j]
+            acc = Fraction(0)
+            for k in range(d):
+                if mode == 1:                       # acc += (q - c) * (q - c)
+                    t = R(q[k] - c[k])
+                    acc = R(acc + t * t) if fused else R(acc + R(t * t))
+                else:                               # acc += (pt * c) + q2 + c2, c2 = c * c
+                    if fused:
+                        s_ = R(c[k] * c[k] + R(pt[k] * c[k] + q2[k]))
+                    else:
+                        s_ = R(R(R(pt[k] * c[k]) + q2[k]) + R(c[k] * c[k]))
+                    acc = R(acc + s_)
+            out[i, j] = float(acc)
+    return out
+
+
+def _hits(acc, eps, prec, mode):
+    v = np.abs(acc) if mode == 2 else acc
+    if prec == 32:
+        return int((np.sqrt(v.astype(np.float32)) <= np.float32(eps)).sum())
+    return int((np.sqrt(v) <= eps).sum())
+
+
+@pytest.mark.parametrize("mode,prec", [(1, 32), (2, 32), (2, 64)])
+def test_same_type_contraction_pinned_by_exact_emulation(mode, prec):
+    rng = np.random.default_rng(40 + mode + prec)
+    x = rng.uniform(0.3, 0.45, size=(34, 19)) * (1.0 + rng.uniform(-1e-3, 1e-3, size=(34, 1)))
+    fused = _emulate_d2(x, mode, prec, True)
+    plain = _emulate_d2(x, mode, prec, False)
+    ft = (lambda v: np.sqrt(np.abs(v).astype(np.float32))) if prec == 32 else (lambda v: np.sqrt(np.abs(v)))
+    sf, sp = ft(fused), ft(plain)
+    # eps at the square roots where the two readings disagree: one counts the pair, the other not
+    split = np.argwhere(sf != sp)
+    assert split.shape[0] >= 4, "the data do not separate the fused and unfused readings"
+    eps_list = sorted({float(min(sf[i, j], sp[i, j])) for i, j in split[:40]})
+    differ = 0
+    for eps in eps_list:
+        want = _hits(fused, eps, prec, mode)
+        differ += want != _hits(plain, eps, prec, mode)
+        assert oracle.reference_total(x, eps, mode, prec, prec) == want, eps
+    assert differ >= 1, "no eps separates the readings: the check could not fail"
 
 
 def test_unsupported_combinations_raise():
