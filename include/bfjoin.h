@@ -139,6 +139,24 @@ int bfj_self_join_device(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t 
                          int64_t row_begin, int64_t row_end,
                          int64_t *d_counts, uint64_t *totals, bfj_stats *stats);
 
+/* bfj_self_join_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the call
+ * (code that runs under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and status codes as bfj_self_join_device; the
+ * pass's working buffers (packed operands, norms, candidate stream, band lists,
+ * counters) come from one file-scope device array of BFJ_NOALLOC_SCRATCH_BYTES that
+ * each GPU holds until the process ends, not from hipMalloc.  Limits: the dataset
+ * must fit that array (about 1.5 KiB per row at d = 128 FP16 besides the candidate
+ * stream and band lists, which take the rest: n up to roughly 16k rows), and a pass
+ * whose candidate stream or band list does not fit returns BFJ_ERR_CAPACITY instead
+ * of growing them.  Calls on one GPU share its array, so the library runs them one
+ * at a time (a per-device lock); bfj_pass_errors does not apply to them. */
+#define BFJ_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int bfj_self_join_device_noalloc(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t d, int64_t ld,
+                                 const double *eps, int32_t n_eps, int32_t dtype,
+                                 int64_t row_begin, int64_t row_end,
+                                 int64_t *d_counts, uint64_t *totals, bfj_stats *stats);
+
 /* The result set instead of counts: every unordered pair {i, j}, i < j, of the n
  * rows with sqrt(d2(i, j)) <= eps (the reference's predicate, kernel_join.cu:194),
  * written to pairs[2k], pairs[2k+1] (host, i < j) in no particular order.  The

@@ -115,6 +115,8 @@ def load_library(path: str = LIB_PATH):
         "bfj_destroy": (ctypes.c_int, [vp]),
         "bfj_self_join": (ctypes.c_int, [vp, vp, i64, i32, i64, f64p, i32, i32, i64, i64, vp, u64p, sp]),
         "bfj_self_join_device": (ctypes.c_int, [vp, vp, i64, i32, i64, f64p, i32, i32, i64, i64, vp, u64p, sp]),
+        "bfj_self_join_device_noalloc": (ctypes.c_int, [vp, vp, i64, i32, i64, f64p, i32, i32, i64, i64, vp, u64p,
+                                                        sp]),
         "bfj_self_join_batch": (ctypes.c_int, [vp, ctypes.POINTER(Dataset), i32, i32, i32, sp]),
         "bfj_self_join_pairs": (ctypes.c_int, [vp, vp, i64, i32, i64, ctypes.c_double, i32, vp, ctypes.c_uint64,
                                                u64p, u64p, sp]),

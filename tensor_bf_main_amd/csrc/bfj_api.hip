@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <atomic>
 #include <memory>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -219,6 +220,8 @@ struct bfj_ctx {
     // buffers); cleared when another pass starts or its rows may be gone.
     std::unique_ptr<struct PassPlan> last;
     DevBuf errbuf;
+    void *scratch = nullptr; // device address of this GPU's no-allocation scratch array
+    int noalloc = 0;         // a no-allocation call is running: buffers come from scratch, never grow
 };
 
 // Bumped whenever a device buffer moves: a captured graph holds raw pointers.
@@ -226,9 +229,40 @@ static std::atomic<uint64_t> g_buf_gen{1};
 // Source of the DevBuf and work-item list ids.
 static std::atomic<uint64_t> g_obj_id{1};
 
+// The no-allocation entry point's device memory: one file-scope array, which every GPU
+// the library runs on holds (its own copy) until the process ends.  In host code the
+// name is only a stand-in: the array's device address on each GPU comes from
+// hipGetSymbolAddress with that GPU current (scratch_address) and is the only
+// address host code ever uses for it.
+#define BFJ_SCRATCH_BYTES ((size_t)BFJ_NOALLOC_SCRATCH_BYTES)
+#define BFJ_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[BFJ_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;                          // guards g_scratch_ptr
+static void *g_scratch_ptr[BFJ_MAX_DEVICES];             // per device: its array's address
+static std::mutex g_scratch_call_mu[BFJ_MAX_DEVICES];    // per device: one no-allocation call at a time
+
+// While a no-allocation call runs on this thread, ensure() carves buffers from its
+// device's scratch array (a bump allocator) instead of calling hipMalloc.
+struct Arena {
+    unsigned char *base;
+    size_t size, used;
+};
+static thread_local Arena *t_arena = nullptr;
+
 static int ensure(DevBuf &b, size_t bytes)
 {
     if (bytes <= b.cap) return BFJ_OK;
+    if (t_arena) {
+        const size_t off = (t_arena->used + 255) & ~(size_t)255;
+        if (off > t_arena->size || bytes > t_arena->size - off)
+            return fail(BFJ_ERR_CAPACITY, "the pass needs more than the %zu bytes of the no-allocation scratch",
+                        t_arena->size);
+        b.p = t_arena->base + off;
+        b.cap = bytes;
+        b.id = g_obj_id.fetch_add(1);
+        t_arena->used = off + bytes;
+        return BFJ_OK;
+    }
     g_buf_gen.fetch_add(1);
     if (b.p) HC(hipFree(b.p));
     b.p = nullptr;
@@ -650,6 +684,12 @@ static int grow_on_overflow(bfj_ctx *ctx, const unsigned long long *ctr, bool *a
 {
     *again = false;
     const unsigned long long band_count = ctr[BFJ_CTR_BAND], chunks = ctr[BFJ_CTR_CHUNKS];
+    if (ctx->noalloc && (chunks > ctx->chunk_capacity || band_count > ctx->band_capacity))
+        return fail(BFJ_ERR_CAPACITY,
+                    "the pass needs %llu candidate chunks and %llu band records; the no-allocation scratch "
+                    "(%zu bytes) holds %u and %llu for this dataset",
+                    chunks, band_count, BFJ_SCRATCH_BYTES, ctx->chunk_capacity,
+                    (unsigned long long)ctx->band_capacity);
     if (chunks > ctx->chunk_capacity) {
         if (chunks > BFJ_MAX_CHUNKS)
             return fail(BFJ_ERR_UNSUPPORTED, "the pass needs %llu candidate chunks (at most %llu)", chunks,
@@ -756,6 +796,91 @@ __global__ void k_synth(double *x, int64_t total, uint64_t seed)
     }
 }
 
+// Every device buffer of a context.
+static std::vector<DevBuf *> all_bufs(bfj_ctx *c)
+{
+    return {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
+            &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
+            &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux,
+            &c->perm, &c->label, &c->ohist, &c->oseg, &c->pivots, &c->pivot_norm, &c->mid_rows,
+            &c->mid_q, &c->mid_band, &c->errbuf};
+}
+
+// This device's scratch address, looked up once per device with the device current.
+static int scratch_address(int device, void **ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= BFJ_MAX_DEVICES)
+        return fail(BFJ_ERR_UNSUPPORTED, "device %d: the no-allocation scratch is kept for %d devices", device,
+                    BFJ_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(BFJ_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        }
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return BFJ_OK;
+}
+
+namespace {
+
+// For the length of a no-allocation call: the context's buffers, capacities and cached
+// work items are set aside and every buffer starts empty (ensure() then carves from the
+// arena); the destructor puts them back.
+struct NoallocScope {
+    bfj_ctx *c;
+    std::vector<DevBuf> saved;
+    uint64_t band_capacity, items_staged_id, items_staged_buf_id, item_cache_id;
+    uint32_t chunk_capacity;
+    std::vector<int64_t> item_key;
+    std::shared_ptr<const std::vector<WorkItem>> item_cache;
+    std::vector<Group> group_cache;
+    explicit NoallocScope(bfj_ctx *ctx) : c(ctx)
+    {
+        for (DevBuf *b : all_bufs(c)) {
+            saved.push_back(*b);
+            *b = DevBuf();
+        }
+        band_capacity = c->band_capacity;
+        chunk_capacity = c->chunk_capacity;
+        items_staged_id = c->items_staged_id;
+        items_staged_buf_id = c->items_staged_buf_id;
+        item_key = c->item_key;
+        item_cache = c->item_cache;
+        item_cache_id = c->item_cache_id;
+        group_cache = c->group_cache;
+        c->last.reset();
+        c->noalloc = 1;
+    }
+    void clear_bufs()
+    {
+        for (DevBuf *b : all_bufs(c)) *b = DevBuf();
+    }
+    ~NoallocScope()
+    {
+        size_t k = 0;
+        for (DevBuf *b : all_bufs(c)) *b = saved[k++];
+        c->band_capacity = band_capacity;
+        c->chunk_capacity = chunk_capacity;
+        c->items_staged_id = items_staged_id;
+        c->items_staged_buf_id = items_staged_buf_id;
+        c->item_key = item_key;
+        c->item_cache = item_cache;
+        c->item_cache_id = item_cache_id;
+        c->group_cache = group_cache;
+        c->last.reset(); // the pass's buffers were scratch: nothing to measure afterwards
+        c->noalloc = 0;
+        t_arena = nullptr;
+    }
+};
+
+} // namespace
+
 // ---------------------------------------------------------------------------
 // C ABI
 extern "C" {
@@ -797,8 +922,14 @@ int bfj_init(int device, bfj_ctx **out)
     }
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) {
-        delete c;
+        bfj_destroy(c);
         return fail(BFJ_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(e));
+    }
+    const int rc = scratch_address(device, &c->scratch);
+    if (rc != BFJ_OK) {
+        const std::string msg = g_err;
+        bfj_destroy(c);
+        return fail(rc, "%s", msg.c_str());
     }
     *out = c;
     return BFJ_OK;
@@ -809,12 +940,7 @@ int bfj_destroy(bfj_ctx *c)
     if (!c) return BFJ_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
-                      &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
-                      &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux,
-                      &c->perm, &c->label, &c->ohist, &c->oseg, &c->pivots, &c->pivot_norm, &c->mid_rows,
-                      &c->mid_q, &c->mid_band, &c->errbuf};
-    for (DevBuf *b : bufs) release(*b);
+    for (DevBuf *b : all_bufs(c)) release(*b);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -877,6 +1003,57 @@ int bfj_self_join_device(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t 
                          uint64_t *totals, bfj_stats *stats)
 {
     RC(common_checks(ctx, d_data, n, eps, n_eps));
+    return join_one(ctx, d_data, n, d, ld, eps, n_eps, dtype, row_begin, row_end, d_counts, totals, stats, 0);
+}
+
+int bfj_self_join_device_noalloc(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t d, int64_t ld,
+                                 const double *eps, int32_t n_eps, int32_t dtype, int64_t row_begin, int64_t row_end,
+                                 int64_t *d_counts, uint64_t *totals, bfj_stats *stats)
+{
+    RC(common_checks(ctx, d_data, n, eps, n_eps));
+    if (!ctx->scratch) return fail(BFJ_ERR_DEVICE, "no scratch address for device %d", ctx->device);
+    std::lock_guard<std::mutex> lk(g_scratch_call_mu[ctx->device]);
+    NoallocScope scope(ctx);
+    Arena arena{(unsigned char *)ctx->scratch, BFJ_SCRATCH_BYTES, 0};
+    t_arena = &arena;
+    if (n > 0) {
+        // Sizing pass (no device work): the buffers every pass of this shape needs, with
+        // the smallest candidate stream and band lists; the rest of the scratch then goes
+        // to those two, 60 % to the candidate stream (FP8 bands hold about 0.6 records
+        // per candidate, in two lists).
+        HostSet h;
+        memset(&h, 0, sizeof h);
+        h.x = d_data;
+        h.n = n;
+        h.d = d;
+        h.ld = ld;
+        h.row_begin = row_begin;
+        h.row_end = row_end;
+        h.n_eps = n_eps;
+        h.out = d_counts;
+        for (int e = 0; e < n_eps; ++e) h.eps[e] = eps[e];
+        PassPlan p;
+        RC(build_plan(ctx, {h}, dtype, p, 0));
+        ctx->chunk_capacity = 64;
+        ctx->band_capacity = 1024;
+        const int rc = stage_plan(ctx, p);
+        if (rc == BFJ_ERR_CAPACITY)
+            return fail(BFJ_ERR_CAPACITY, "n = %lld, d = %d needs more than the %zu bytes of the no-allocation scratch",
+                        (long long)n, d, BFJ_SCRATCH_BYTES);
+        RC(rc);
+        const size_t fixed = arena.used + 64 * 1024; // + alignment of every buffer
+        const size_t rest = BFJ_SCRATCH_BYTES > fixed ? BFJ_SCRATCH_BYTES - fixed : 0;
+        const size_t lists = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? 2 : 1;
+        const size_t chunks = rest / 10 * 6 / ((size_t)BFJ_CHUNK * sizeof(Candidate) + 4);
+        const size_t band = rest / 10 * 4 / (lists * sizeof(BandEntry));
+        if (chunks < 64 || band < 1024)
+            return fail(BFJ_ERR_CAPACITY, "n = %lld, d = %d leaves too little of the %zu-byte no-allocation scratch "
+                        "for the candidate stream", (long long)n, d, BFJ_SCRATCH_BYTES);
+        ctx->chunk_capacity = (uint32_t)std::min<size_t>(chunks, BFJ_MAX_CHUNKS);
+        ctx->band_capacity = band;
+        scope.clear_bufs();
+        arena.used = 0;
+    }
     return join_one(ctx, d_data, n, d, ld, eps, n_eps, dtype, row_begin, row_end, d_counts, totals, stats, 0);
 }
 
