@@ -328,6 +328,50 @@ def test_forced_overflow_grows_and_repeats(ctx, dtype):
     plan.close()
 
 
+def _adversarial(kind, seed):
+    """Data built to stress the error bound (each is checked for every dtype against the
+    oracle, with eps at the FP64 distances of real pairs)."""
+    rng = np.random.default_rng(seed)
+    if kind == "offset":            # a large common offset: centring must not lose the distances
+        x = 1e6 + rng.random((2500, 32))
+    elif kind == "row_scales":      # row norms spread over 2^-40 .. 2^40
+        x = rng.standard_normal((2500, 48)) * np.exp2(rng.uniform(-40, 40, (2500, 1)))
+    elif kind == "subnormal":       # all but a few rows subnormal in FP16 / e4m3 after the scale
+        x = rng.random((2500, 40)) * 1e-7
+        x[:5] = rng.random((5, 40))
+    elif kind == "dups_1e-12":      # duplicates with 1e-12 jitter: distances far below any cast's resolution
+        base = rng.random((1200, 64))
+        x = np.concatenate([base, base + rng.standard_normal(base.shape) * 1e-12,
+                            base[:300] + rng.standard_normal((300, 64)) * 1e-12])
+    elif kind == "cancel_d256":     # rows that nearly cancel: tiny distances between large rows
+        c = rng.random((1, 256)) * 100.0
+        x = c + rng.standard_normal((2000, 256)) * 1e-4
+    else:
+        raise ValueError(kind)
+    n = x.shape[0]
+    # eps at the exact FP64 distance of each of three rows to its nearest other row (the
+    # pairs then lie inside every cast's band), and two quantiles
+    exact = []
+    for i in rng.integers(0, n, 3):
+        d2 = oracle.pair_d2(x, np.full(n, i), np.arange(n))
+        d2[i] = np.inf
+        exact.append(float(np.sqrt(d2.min())))
+    return x, exact + wl.eps_quantile(x, [0.0005, 0.01], 300, seed)
+
+
+@pytest.mark.parametrize("kind", ["offset", "row_scales", "subnormal", "dups_1e-12", "cancel_d256"])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_adversarial_data_stays_exact(ctx, kind, dtype):
+    x, eps = _adversarial(kind, 100 + len(kind))
+    res = _check(ctx, x, eps, dtype)
+    e = ctx.pass_errors(max_candidates=0, sample_tiles=512)
+    assert e["violations"] == 0, e
+    if dtype != "fp64" and e["tile_pairs"]:
+        assert e["max_acc_ratio"] <= 1.0 and e["max_quant_ratio"] <= 1.0, e
+    # the first three eps are distances of real pairs: each holds at least that pair
+    assert all(int(res.totals[k]) >= x.shape[0] + 2 for k in range(3))
+
+
 def test_device_resident_path(ctx):
     x, eps = wl.config(1, n=5000)
     xd = ctx.to_device(x)
