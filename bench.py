@@ -286,7 +286,7 @@ def batch_workload(args, ctx):
            "per_dataset_launches_ms": t_each * 1e3, "per_dataset_pairs_per_s": pairs / t_each,
            "graph_speedup_vs_per_dataset": t_each / t_graph, "plan_create_s": t_plan,
            "band_pairs": int(st["band_pairs"]), "fp64_refine_pairs": int(st["fp64_pairs"]),
-           "candidates": int(st["candidates"]), "join_counted": int(st["join_counted"]),
+           "candidates": int(st["candidates"]),
            "stage_ms": {k: st[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
            "roofline": roofline(flops, st["ms_join"], args.dtype), "errors": errors,
            "mismatches_graph_vs_eager": mismatches}

@@ -35,8 +35,9 @@
 extern "C" {
 #endif
 
-/* 2: bfj_stats gained fp64_pairs.  3: bfj_stats gained candidates and join_counted;
-   bfj_set_capacity, bfj_pass_errors.  (A larger struct: callers built against an
+/* 2: bfj_stats gained fp64_pairs.  3: bfj_stats lost max_abs_d2_err (see bfj_pass_errors)
+   and gained candidates; bfj_set_capacity, bfj_fail_allocation, bfj_pass_errors and
+   bfj_self_join_device_noalloc.  (Another struct layout: callers built against an
    earlier version must be rebuilt.) */
 #define BFJ_ABI_VERSION 3
 
@@ -79,10 +80,8 @@ typedef struct bfj_stats {
     double ms_total;         /* device time of the whole pass */
     uint64_t fp64_pairs;     /* band pairs re-evaluated in FP64: those the middle stage (FP16/BF16/FP8
                                 passes) left undecided; band_pairs for FP64 passes */
-    uint64_t candidates;     /* pair records the join streamed to the band tests (pairs it could
-                                neither reject nor count itself) */
-    uint64_t join_counted;   /* pairs the join counted itself (certain for every eps), 0 when the
-                                pass streams every candidate */
+    uint64_t candidates;     /* pair records the join streamed to the band tests (the pairs it
+                                could not reject) */
 } bfj_stats;
 
 /* One dataset of a batch.  data/counts are host or device pointers depending on
@@ -218,6 +217,12 @@ int bfj_set_reorder(bfj_ctx *ctx, int32_t enable);
  * on these values, only the number of passes does.  Used to pre-size the buffers
  * for a known workload, or to force the grow-and-repeat path in tests. */
 int bfj_set_capacity(bfj_ctx *ctx, uint64_t candidate_chunks, uint64_t band_records);
+
+/* Fault injection for tests of the error paths (no counterpart in the reference):
+ * the countdown-th device allocation the library makes from now on (over every context
+ * of the process) fails as if the device were out of memory, and the call that made it
+ * returns BFJ_ERR_NOMEM; 0 turns it off.  A context stays usable after such a failure. */
+int bfj_fail_allocation(int64_t countdown);
 
 /* Accuracy of the last pass run on this context (no counterpart in the reference,
  * which reports no error): sigma^2 S_hat, the squared distance the pass's dot

@@ -33,7 +33,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libbfj_oracle.so")
+# BFJ_ORACLE_LIB selects another build of bfj_oracle.c (the sanitizer build in tests).
+_LIB_PATH = os.environ.get("BFJ_ORACLE_LIB") or os.path.join(_HERE, "libbfj_oracle.so")
 _REF_IMPORT = os.path.join(_HERE, "_ref", "ref_import")
 
 _lib = None

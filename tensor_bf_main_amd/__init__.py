@@ -55,7 +55,6 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("fp64_pairs", ctypes.c_uint64),
         ("candidates", ctypes.c_uint64),
-        ("join_counted", ctypes.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -128,6 +127,7 @@ def load_library(path: str = LIB_PATH):
         "bfj_set_tile_rows": (ctypes.c_int, [vp, i32, i32]),
         "bfj_set_reorder": (ctypes.c_int, [vp, i32]),
         "bfj_set_capacity": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint64]),
+        "bfj_fail_allocation": (ctypes.c_int, [i64]),
         "bfj_pass_errors": (ctypes.c_int, [vp, ctypes.c_uint64, i64, ctypes.c_uint64, ctypes.POINTER(Errors)]),
         "bfj_tile_rows_options": (ctypes.c_int, [i32, i32, ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]),
         "bfj_buffer_alloc": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(vp)]),

@@ -249,6 +249,10 @@ struct Arena {
 };
 static thread_local Arena *t_arena = nullptr;
 
+// Allocation-failure injection (bfj_fail_allocation): the countdown-th device allocation
+// from the time it was set fails as if the device were out of memory.
+static std::atomic<int64_t> g_fail_countdown{0};
+
 static int ensure(DevBuf &b, size_t bytes)
 {
     if (bytes <= b.cap) return BFJ_OK;
@@ -269,6 +273,8 @@ static int ensure(DevBuf &b, size_t bytes)
     b.cap = 0;
     b.id = 0;
     size_t want = bytes + bytes / 8 + 256;
+    if (g_fail_countdown.load() > 0 && g_fail_countdown.fetch_sub(1) == 1)
+        return fail(BFJ_ERR_NOMEM, "hipMalloc(%zu bytes) failed: allocation failure injected", want);
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
         b.p = nullptr;
@@ -1466,6 +1472,13 @@ int bfj_autotune(bfj_ctx *ctx, int32_t dtype, int32_t d, int64_t n, int32_t *til
     ctx->last.reset(); // its rows are gone
     ctx->tile_rows[dtype][kc] = best;
     if (tile_rows_out) *tile_rows_out = best;
+    return BFJ_OK;
+}
+
+int bfj_fail_allocation(int64_t countdown)
+{
+    if (countdown < 0) return fail(BFJ_ERR_INVALID, "countdown < 0");
+    g_fail_countdown.store(countdown);
     return BFJ_OK;
 }
 

@@ -372,6 +372,51 @@ def test_adversarial_data_stays_exact(ctx, kind, dtype):
     assert all(int(res.totals[k]) >= x.shape[0] + 2 for k in range(3))
 
 
+def test_allocation_failure_is_reported_and_the_context_recovers():
+    """Every device allocation of a context's first pass, failed in turn (the k-th one
+    from the start of the call): the call returns BFJ_ERR_NOMEM with the reason, and the
+    same context then runs the pass correctly.  The same for a plan's creation."""
+    import tensor_bf_main_amd as bfj_
+    lib = bfj_.load_library()
+    x, eps = wl.config(3, n=4000)
+    oc, ot = oracle.join(x, eps)
+    failures = 0
+    try:
+        for k in range(1, 80):
+            with bfj_.Context(0) as c:
+                assert lib.bfj_fail_allocation(k) == 0
+                try:
+                    c.self_join(x, eps, dtype="fp16")
+                    hit = False
+                except bfj_.JoinError as err:
+                    assert err.code == 3 and "injected" in str(err), str(err)
+                    hit = True
+                lib.bfj_fail_allocation(0)
+                res = c.self_join(x, eps, dtype="fp16")
+                np.testing.assert_array_equal(res.counts, oc)
+                np.testing.assert_array_equal(res.totals, ot)
+                failures += hit
+            if not hit:
+                break
+        assert failures >= 20, failures  # a first pass allocates every buffer of the context
+        sets = wl.batch(n_sets=3, seed=82, n_min=500, n_max=1500)
+        with bfj_.Context(0) as c:
+            dev = [{"data": c.to_device(s["data"]), "eps": s["eps"],
+                    "counts": c.empty((len(s["eps"]), s["data"].shape[0]), np.int64)} for s in sets]
+            lib.bfj_fail_allocation(3)
+            with pytest.raises(bfj_.JoinError) as ei:
+                c.plan(dev, dtype="fp16")
+            assert ei.value.code == 3
+            lib.bfj_fail_allocation(0)
+            plan = c.plan(dev, dtype="fp16")
+            plan.run()
+            for dv, s in zip(dev, sets):
+                np.testing.assert_array_equal(dv["counts"].numpy(), oracle.join(s["data"], s["eps"])[0])
+            plan.close()
+    finally:
+        lib.bfj_fail_allocation(0)
+
+
 def test_device_resident_path(ctx):
     x, eps = wl.config(1, n=5000)
     xd = ctx.to_device(x)
