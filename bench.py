@@ -288,7 +288,7 @@ def batch_workload(args, ctx):
            "band_pairs": int(st["band_pairs"]), "fp64_refine_pairs": int(st["fp64_pairs"]),
            "candidates": int(st["candidates"]),
            "stage_ms": {k: st[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
-           "roofline": roofline(flops, st["ms_join"], args.dtype), "errors": errors,
+           "roofline": roofline(flops, st["ms_join"], args.dtype), "errors": errors, "tiles": plan.tiles(),
            "mismatches_graph_vs_eager": mismatches}
     if not args.skip_cpu:
         from oracle import oracle

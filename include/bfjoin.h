@@ -36,8 +36,8 @@ extern "C" {
 #endif
 
 /* 2: bfj_stats gained fp64_pairs.  3: bfj_stats lost max_abs_d2_err (see bfj_pass_errors)
-   and gained candidates; bfj_set_capacity, bfj_fail_allocation, bfj_pass_errors and
-   bfj_self_join_device_noalloc.  (Another struct layout: callers built against an
+   and gained candidates; bfj_set_capacity, bfj_fail_allocation, bfj_pass_errors,
+   bfj_plan_tiles and bfj_self_join_device_noalloc.  (Another struct layout: callers built against an
    earlier version must be rebuilt.) */
 #define BFJ_ABI_VERSION 3
 
@@ -186,9 +186,18 @@ int bfj_reference_mode_join(bfj_ctx *ctx, const double *data, int64_t n, int32_t
 int bfj_self_join_batch(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32_t dtype,
                         int32_t device_resident, bfj_stats *stats);
 
-/* Plans: all buffers allocated, tiles tuned and the whole pass captured in one
- * hipGraph at creation; bfj_plan_run replays it.  Datasets are device-resident. */
+/* Plans: all buffers allocated, join shapes tuned (bfj_plan_tiles) and the whole pass
+ * captured in one hipGraph at creation; bfj_plan_run replays it.  Datasets are
+ * device-resident. */
 int bfj_plan_create(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32_t dtype, bfj_plan **plan);
+/* The join shape a plan runs with, per K-chunk group (datasets whose d pads to the same
+ * number of K-chunks): kc, rows per wavefront, resident blocks per CU and candidate
+ * panel bytes.  bfj_plan_create picks them for every group that has no shape set with
+ * bfj_set_tile_rows: it times whole passes over a seeded sample of the group's own
+ * datasets at their own eps, for every built row count x blocks per CU, then two
+ * other panel sizes.  Writes up to max_groups entries; *n_groups gets the count. */
+int bfj_plan_tiles(const bfj_plan *plan, int32_t *kc, int32_t *rows, int32_t *blocks_per_cu, int32_t *panel_bytes,
+                   int32_t max_groups, int32_t *n_groups);
 int bfj_plan_run(bfj_plan *plan, bfj_stats *stats);
 int bfj_plan_destroy(bfj_plan *plan);
 

@@ -123,6 +123,8 @@ def load_library(path: str = LIB_PATH):
         "bfj_plan_create": (ctypes.c_int, [vp, ctypes.POINTER(Dataset), i32, i32, ctypes.POINTER(vp)]),
         "bfj_plan_run": (ctypes.c_int, [vp, sp]),
         "bfj_plan_destroy": (ctypes.c_int, [vp]),
+        "bfj_plan_tiles": (ctypes.c_int, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                          ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]),
         "bfj_autotune": (ctypes.c_int, [vp, i32, i32, i64, ctypes.POINTER(i32)]),
         "bfj_set_tile_rows": (ctypes.c_int, [vp, i32, i32]),
         "bfj_set_reorder": (ctypes.c_int, [vp, i32]),
@@ -457,6 +459,14 @@ class Plan:
 
     def totals(self):
         return [t.copy() for (_, _, t, _) in self._keep]
+
+    def tiles(self) -> list[dict]:
+        """The join shape of each K-chunk group (bfj_plan_tiles)."""
+        arrs = [(ctypes.c_int32 * 33)() for _ in range(4)]
+        n = ctypes.c_int32(0)
+        _check(self.ctx.lib.bfj_plan_tiles(self._h, *arrs, 33, ctypes.byref(n)))
+        keys = ("kc", "rows", "blocks_per_cu", "panel_bytes")
+        return [{k: int(a[g]) for k, a in zip(keys, arrs)} for g in range(min(n.value, 33))]
 
     def close(self):
         if getattr(self, "_h", None):

@@ -188,6 +188,8 @@ struct DevBuf {
 
 struct Group {
     int kc, rows, lds;
+    int cap;         // resident join blocks per CU of the persistent grid (0: as many as fit)
+    int panel_bytes; // candidate fragment bytes per work item
     int32_t item0, n_items;
 };
 
@@ -205,7 +207,9 @@ struct bfj_ctx {
     uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
     int emit_pairs = 0;
     uint32_t chunk_capacity = 0; // candidate chunks of BFJ_CHUNK records
-    int tile_rows[4][33] = {}; // [dtype][kc] forced/tuned R (0 = heuristic)
+    int tile_rows[4][33] = {};  // [dtype][kc] forced/tuned R (0 = heuristic)
+    int tile_cap[4][33] = {};   // [dtype][kc] tuned resident blocks per CU (0 = as many as fit)
+    int tile_panel[4][33] = {}; // [dtype][kc] tuned panel bytes (0 = BFJ_PANEL_BYTES)
     // Work items of the last plan shape (repeated calls on the same shapes reuse them,
     // and skip the upload while the device copy is current).  Item lists are known by
     // a unique id, never by address: a list freed with its plan can be replaced by a
@@ -418,8 +422,10 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         if (!bfj_join_has(dtype, kc, rows)) return fail(BFJ_ERR_UNSUPPORTED, "no tile for dtype %d kc %d", dtype, kc);
         g.rows = rows;
         g.lds = lds_for(dtype, kc, rows, n_eps_max);
+        g.cap = ctx->tile_cap[dtype][kc];
+        g.panel_bytes = ctx->tile_panel[dtype][kc] ? ctx->tile_panel[dtype][kc] : BFJ_PANEL_BYTES;
         groups.push_back(g);
-        key.insert(key.end(), {kc, rows, g.lds});
+        key.insert(key.end(), {kc, rows, g.lds, g.cap, g.panel_bytes});
     }
     for (auto &s : p.sets) key.insert(key.end(), {s.kc, s.npad, s.sym, s.row_begin, s.row_end});
     if (ctx->item_cache && key == ctx->item_key) {
@@ -438,7 +444,7 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         // in flight on an XCD sweep the same panel and its tiles are fetched into that
         // XCD's L2 once instead of once per strip.  Datasets largest first.
         const int64_t tile_bytes = (int64_t)kc * BFJ_WAVE * FB;
-        const int panel = (int)std::max<int64_t>(32, (int64_t)BFJ_PANEL_BYTES / tile_bytes / 16 * 16);
+        const int panel = (int)std::max<int64_t>(32, (int64_t)g.panel_bytes / tile_bytes / 16 * 16);
         const int strip = BFJ_JWAVES * rows / 16;
         std::vector<size_t> order;
         for (size_t si = 0; si < p.sets.size(); ++si)
@@ -641,7 +647,8 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
         L.kc = g.kc;
         L.rows = g.rows;
         L.lds_bytes = g.lds;
-        const int occ = bfj_join_max_blocks_per_cu(p.dtype, g.kc, g.rows, g.lds);
+        int occ = bfj_join_max_blocks_per_cu(p.dtype, g.kc, g.rows, g.lds);
+        if (g.cap > 0) occ = std::min(occ, g.cap);
         const int persistent = std::max(1, occ) * ctx->n_cu;
         L.grid_blocks = std::min(persistent, g.n_items);
         if (L.grid_blocks < 1) L.grid_blocks = 1;
@@ -1238,6 +1245,89 @@ int bfj_self_join_batch(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, i
     return BFJ_OK;
 }
 
+// ---- tile search -------------------------------------------------------------------
+// The shape of one K-chunk group of a batch, searched on a seeded sample of the group's
+// own datasets (about 3e10 pairs: small passes are ruled by fixed costs, not the join's
+// shape) at their own eps.  Each shape is timed as whole passes (device time of prep,
+// join, band tests and counting; the fastest of two).  The search starts from the
+// heuristic's shape and moves only for a gain above 2 %: R x resident join blocks per
+// CU, then the panel size.
+static int tune_group(bfj_ctx *ctx, const std::vector<HostSet> &group, int dtype, int kc)
+{
+    std::vector<size_t> order(group.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+    uint64_t seed = 0x5eed0000ull + (uint64_t)kc;
+    for (size_t k = order.size(); k > 1; --k) { // Fisher-Yates with splitmix64
+        uint64_t z = (seed += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        std::swap(order[k - 1], order[(size_t)(z % k)]);
+    }
+    std::vector<HostSet> sample;
+    double pairs = 0.0;
+    int n_eps_max = 1;
+    for (size_t k : order) {
+        if (!sample.empty() && pairs >= 3e10) break;
+        HostSet h = group[k];
+        h.out = nullptr; // tuning passes write no counts
+        sample.push_back(h);
+        pairs += (double)h.n * (double)h.n;
+        n_eps_max = std::max(n_eps_max, (int)h.n_eps);
+    }
+    auto time_shape = [&](int rows, int cap, int panel, double *ms) -> int {
+        ctx->tile_rows[dtype][kc] = rows;
+        ctx->tile_cap[dtype][kc] = cap;
+        ctx->tile_panel[dtype][kc] = panel;
+        PassPlan p;
+        RC(build_plan(ctx, sample, dtype, p, 0));
+        RC(stage_plan(ctx, p));
+        *ms = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            bfj_stats st;
+            RC(run_pass(ctx, p, nullptr, &st));
+            *ms = std::min(*ms, st.ms_total);
+        }
+        return BFJ_OK;
+    };
+    // The heuristic's shape first (rows 0: heuristic_rows; all blocks that fit; default panel).
+    bool reordered = ctx->reorder != 0;
+    for (const HostSet &h : sample) reordered = reordered && h.n >= 512;
+    int best_r = heuristic_rows(dtype, kc, n_eps_max, reordered), best_cap = 0, best_panel = BFJ_PANEL_BYTES;
+    double best_ms;
+    RC(time_shape(best_r, 0, BFJ_PANEL_BYTES, &best_ms));
+    for (int r : kRowOptions) {
+        if (!bfj_join_has(dtype, kc, r)) continue;
+        const int occ = std::max(1, bfj_join_max_blocks_per_cu(dtype, kc, r, lds_for(dtype, kc, r, n_eps_max)));
+        std::vector<int> caps = {0}; // as many blocks as fit, then fewer (down to 2)
+        for (int c : {4, 3, 2})
+            if (c < occ) caps.push_back(c);
+        for (int cap : caps) {
+            if (r == best_r && cap == 0) continue; // timed above
+            double ms;
+            RC(time_shape(r, cap, BFJ_PANEL_BYTES, &ms));
+            if (ms < 0.98 * best_ms) {
+                best_ms = ms;
+                best_r = r;
+                best_cap = cap;
+            }
+        }
+    }
+    for (int panel : {BFJ_PANEL_BYTES / 2, BFJ_PANEL_BYTES * 2}) {
+        double ms;
+        RC(time_shape(best_r, best_cap, panel, &ms));
+        if (ms < 0.98 * best_ms) {
+            best_ms = ms;
+            best_panel = panel;
+        }
+    }
+    ctx->tile_rows[dtype][kc] = best_r;
+    ctx->tile_cap[dtype][kc] = best_cap;
+    ctx->tile_panel[dtype][kc] = best_panel;
+    ctx->last.reset();
+    return BFJ_OK;
+}
+
 // ---- plans: one hipGraph per batch ---------------------------------------------
 struct bfj_plan {
     bfj_ctx *ctx = nullptr;
@@ -1282,21 +1372,21 @@ int bfj_plan_create(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32
     std::vector<HostSet> hs;
     std::vector<size_t> xoff, ooff;
     RC(batch_hostsets(ctx, sets, n_sets, 1, hs, xoff, ooff));
-    // Tile shapes: every K-chunk group that has no tuned tile yet is autotuned on a
-    // synthetic problem of its largest dimension (capped at 32768 rows).
-    if (dtype >= BFJ_FP64 && dtype <= BFJ_FP8)
+    // Tile shapes: every K-chunk group without a tuned or forced shape is tuned on a
+    // sample of its own datasets, at their own eps (tune_group).
+    if (dtype >= BFJ_FP64 && dtype <= BFJ_FP8) {
+        std::vector<int> done;
         for (const HostSet &h : hs) {
-            const int kc = (h.d >= 1 && h.d <= BFJ_MAX_DIM) ? pick_kc(dtype, h.d) : 0;
-            if (!kc || ctx->tile_rows[dtype][kc]) continue;
-            int dmax = 1;
-            int64_t nmax = 256;
+            if (validate(h, dtype) != BFJ_OK || h.n == 0) continue; // build_plan reports it
+            const int kc = pick_kc(dtype, h.d);
+            if (ctx->tile_rows[dtype][kc] || std::find(done.begin(), done.end(), kc) != done.end()) continue;
+            done.push_back(kc);
+            std::vector<HostSet> group;
             for (const HostSet &o : hs)
-                if (o.d >= 1 && o.d <= BFJ_MAX_DIM && pick_kc(dtype, o.d) == kc) {
-                    dmax = std::max(dmax, o.d);
-                    nmax = std::max(nmax, o.n);
-                }
-            RC(bfj_autotune(ctx, dtype, dmax, std::min<int64_t>(nmax, 32768), nullptr));
+                if (validate(o, dtype) == BFJ_OK && o.n > 0 && pick_kc(dtype, o.d) == kc) group.push_back(o);
+            RC(tune_group(ctx, group, dtype, kc));
         }
+    }
     bfj_plan *pl = new bfj_plan();
     pl->ctx = ctx;
     pl->sets.assign(sets, sets + n_sets);
@@ -1345,6 +1435,23 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
         RC(capture(pl));
     }
     return fail(BFJ_ERR_INTERNAL, "candidate stream / refine list keep overflowing");
+}
+
+int bfj_plan_tiles(const bfj_plan *pl, int32_t *kc, int32_t *rows, int32_t *blocks_per_cu, int32_t *panel_bytes,
+                   int32_t max_groups, int32_t *n_groups)
+{
+    if (!pl || !n_groups) return fail(BFJ_ERR_INVALID, "plan or n_groups is NULL");
+    const std::vector<Group> &gs = pl->pass.groups;
+    *n_groups = (int32_t)gs.size();
+    for (int32_t k = 0; k < (int32_t)gs.size() && k < max_groups; ++k) {
+        const Group &g = gs[(size_t)k];
+        const int occ = bfj_join_max_blocks_per_cu(pl->pass.dtype, g.kc, g.rows, g.lds);
+        if (kc) kc[k] = g.kc;
+        if (rows) rows[k] = g.rows;
+        if (blocks_per_cu) blocks_per_cu[k] = g.cap > 0 ? std::min(g.cap, occ) : occ;
+        if (panel_bytes) panel_bytes[k] = g.panel_bytes;
+    }
+    return BFJ_OK;
 }
 
 int bfj_plan_destroy(bfj_plan *pl)
@@ -1430,7 +1537,11 @@ int bfj_set_tile_rows(bfj_ctx *ctx, int32_t dtype, int32_t tile_rows)
 {
     if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
     if (dtype < BFJ_FP64 || dtype > BFJ_FP8) return fail(BFJ_ERR_INVALID, "unknown dtype %d", dtype);
-    for (int kc = 0; kc <= 32; ++kc) ctx->tile_rows[dtype][kc] = tile_rows;
+    for (int kc = 0; kc <= 32; ++kc) {
+        ctx->tile_rows[dtype][kc] = tile_rows;
+        ctx->tile_cap[dtype][kc] = 0;
+        ctx->tile_panel[dtype][kc] = 0;
+    }
     return BFJ_OK;
 }
 

@@ -229,6 +229,12 @@ def test_plan_graph_replays(ctx):
             "counts": ctx.empty((len(s["eps"]), s["data"].shape[0]), np.int64)} for s in sets]
     want = [oracle.join(s["data"], s["eps"]) for s in sets]
     plan = ctx.plan(dev, dtype="fp16")
+    # The shapes the plan tuned for its K-chunk groups (on its own datasets) are built ones.
+    tiles = plan.tiles()
+    assert tiles and len({t["kc"] for t in tiles}) == len(tiles)
+    for t in tiles:
+        assert t["rows"] in (16, 32, 64, 96, 128) and t["blocks_per_cu"] >= 1
+        assert t["panel_bytes"] in (1 << 19, 1 << 20, 1 << 21)
     for rep in range(3):
         plan.run()
         for dv, tot, (oc, ot) in zip(dev, plan.totals(), want):
