@@ -584,6 +584,16 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.band = (BandEntry *)ctx->band.p;
     a.band_count = ctr + BFJ_CTR_BAND;
     a.band_capacity = ctx->band_capacity;
+    a.band_records = ctr + BFJ_CTR_BANDREC;
+    // k_classify's wavefronts reserve band slots in blocks; a block as large as a
+    // quarter of the capacity spread over all of them (64..1024 slots) bounds the
+    // holes its partly filled last blocks leave.
+    {
+        const uint64_t share = ctx->band_capacity / (4ull * BFJ_CLASSIFY_WAVES);
+        uint32_t blk = 64;
+        while (blk < 1024 && (uint64_t)blk * 2 <= share) blk *= 2;
+        a.band_block = blk;
+    }
     a.mid = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? 1 : 0;
     a.mid_fp32 = p.dtype != BFJ_FP8 ? 1 : 0;
     a.mid_rows = (unsigned char *)ctx->mid_rows.p;
@@ -735,8 +745,8 @@ static int collect_pass(bfj_ctx *ctx, const PassPlan &p, const unsigned long lon
     if (!stats) return BFJ_OK;
     memset(stats, 0, sizeof *stats);
     stats->pairs = p.pairs;
-    stats->band_pairs = ctr[BFJ_CTR_BAND];
-    stats->fp64_pairs = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? ctr[BFJ_CTR_MID] : ctr[BFJ_CTR_BAND];
+    stats->band_pairs = ctr[BFJ_CTR_BANDREC];
+    stats->fp64_pairs = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? ctr[BFJ_CTR_MID] : ctr[BFJ_CTR_BANDREC];
     stats->candidates = ctr[BFJ_CTR_CANDS];
     stats->band_capacity = ctx->band_capacity;
     stats->band_retries = retries;

@@ -80,6 +80,9 @@ struct BandEntry {
     float z;        // Z value of the pair from the join kernel (kernel space)
     uint32_t meta;  // bits 0..7: eps in the band; bit 15: counts for row j too; 16..31: set index
 };
+// A band-list entry with no eps in the band (meta bits 0..7 clear) is a hole: the
+// unused tail of a block that k_classify reserved.  Consumers skip holes.
+__host__ __device__ inline bool bfj_band_live(const BandEntry &e) { return (e.meta & 0xffu) != 0u; }
 
 // A join work item: one strip of 4R query rows of one dataset against one panel of
 // candidate blocks [cb0, cb1).
@@ -122,11 +125,13 @@ struct JoinArgs {
 // Counters of a pass: one buffer zeroed at the start of every pass, 64-bit slots.
 // Every count that can grow with the number of pairs is 64-bit, so none can wrap
 // and hide an overflow from the host (which compares them with the capacities).
-#define BFJ_CTR_BAND 0   // band-list records reserved by k_classify
+#define BFJ_CTR_BAND 0   // band-list slots reserved by k_classify (records and holes)
 #define BFJ_CTR_CHUNKS 1 // candidate chunks handed out by the join
 #define BFJ_CTR_PAIRS 2  // pairs found in pair-list mode
 #define BFJ_CTR_MID 3    // records the middle stage leaves for the FP64 refine
 #define BFJ_CTR_CANDS 4  // candidate records the join wrote (statistics)
+#define BFJ_CTR_BANDREC 5 // band records k_classify wrote (statistics)
+#define BFJ_CLASSIFY_WAVES (2048 * 4) // k_classify's grid: 2048 blocks of 4 wavefronts
 #define BFJ_CTR_ITEMS 8  // from this slot on: 32-bit work-queue heads, one per join launch
 #define BFJ_CTR_SLOTS 32 // 256 bytes
 
@@ -170,8 +175,10 @@ struct PassArgs {
     uint32_t *cnt;
     int64_t cnt_len;
     BandEntry *band;
-    unsigned long long *band_count;
+    unsigned long long *band_count;    // slots reserved: k_classify takes blocks of band_block
     uint64_t band_capacity;
+    uint32_t band_block;               // slots per k_classify reservation (a power of two, 64..1024)
+    unsigned long long *band_records;  // band records written (statistics)
     // Low-precision passes: a middle stage between classify and the FP64 refine
     // (refine.hip) on higher-precision copies of the rows: FP16 for FP8 passes, FP32
     // for FP16/BF16 passes.

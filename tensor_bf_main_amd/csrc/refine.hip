@@ -276,6 +276,70 @@ __device__ inline Decided decide_fetched(const Fetched &f, const Set &s, const B
 #define BFJ_CLS_GROUPS 4 // 64-record groups of a chunk whose loads are in flight together
 #endif
 
+// The band-list block a wavefront of k_classify is filling.  A wavefront reserves
+// a.band_block slots at a time (one atomic per block on the shared counter instead
+// of one per chunk, and no pass that waits for it per chunk) and writes its band
+// records into them in order; at the end it marks the rest of its last block as
+// holes.  base and fill are wave-uniform.
+struct BandCursor {
+    unsigned long long base; // first slot of the block (~0: none yet)
+    uint32_t fill;           // slots of it written
+    unsigned long long records;
+};
+
+__device__ inline unsigned long long band_reserve(const PassArgs &a, int lane)
+{
+    unsigned long long b = 0;
+    if (lane == 0) b = atomicAdd(a.band_count, (unsigned long long)a.band_block);
+    return __shfl(b, 0);
+}
+
+// Appends the entry of every lane with `take` (ranks in lane order), opening a new
+// block when this one is full; band_block >= 64, so one group needs at most one new
+// block.  Slots past the capacity are reserved but not written (the host sees the
+// counter and repeats the pass).  All lanes take part.
+__device__ inline void band_append(const PassArgs &a, BandCursor &bc, bool take, const BandEntry &be, int lane)
+{
+    const unsigned long long bb = __ballot(take);
+    if (!bb) return;
+    const uint32_t nb = (uint32_t)__popcll(bb);
+    if (bc.base == ~0ull) {
+        bc.base = band_reserve(a, lane);
+        bc.fill = 0;
+    }
+    const uint32_t room = a.band_block - bc.fill;
+    unsigned long long next = ~0ull;
+    if (nb > room) next = band_reserve(a, lane);
+    if (take) {
+        const uint32_t k = (uint32_t)__popcll(bb & ((1ull << lane) - 1ull));
+        const unsigned long long idx = k < room ? bc.base + bc.fill + k : next + (k - room);
+        if (idx < a.band_capacity) a.band[idx] = be;
+    }
+    if (nb > room) {
+        bc.base = next;
+        bc.fill = nb - room;
+    } else {
+        bc.fill += nb;
+    }
+    bc.records += nb;
+}
+
+// The end of a wavefront's part: holes in the rest of its block, and its record count.
+__device__ inline void band_close(const PassArgs &a, const BandCursor &bc, int lane)
+{
+    if (bc.base != ~0ull) {
+        BandEntry hole;
+        hole.i = hole.j = 0;
+        hole.z = 0.f;
+        hole.meta = 0u;
+        for (uint32_t t = bc.fill + lane; t < a.band_block; t += BFJ_WAVE) {
+            const unsigned long long idx = bc.base + t;
+            if (idx < a.band_capacity) a.band[idx] = hole;
+        }
+    }
+    if (lane == 0 && bc.records) atomicAdd(a.band_records, bc.records);
+}
+
 // body(decided, dataset, group) for the groups g0 + k, k < BFJ_CLS_GROUPS, with bit k
 // of gmask (uniform).  When every record of these groups is from one dataset (the
 // common case: a chunk is filled by one join wavefront), its descriptor and scalars
@@ -313,7 +377,6 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
 {
     constexpr int G = BFJ_CLS_GROUPS;
     const int lane = threadIdx.x & 63;
-    const unsigned long long below = (1ull << lane) - 1ull;
     __shared__ CountTable tabs[BFJ_BLOCK / BFJ_WAVE];
     CountTable &tab = tabs[threadIdx.x >> 6];
     const bool use_tab = a.cnt_len < (int64_t)BFJ_CTAB_EMPTY;
@@ -321,6 +384,10 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
     const unsigned long long chunks_used = *a.chunk_count;
     const uint32_t n_chunks = chunks_used < a.chunk_capacity ? (uint32_t)chunks_used : a.chunk_capacity;
     unsigned long long records_read = 0;
+    BandCursor bc;
+    bc.base = ~0ull;
+    bc.fill = 0;
+    bc.records = 0;
     const uint32_t wave_id = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
     const uint32_t n_waves = gridDim.x * (BFJ_BLOCK / BFJ_WAVE);
     for (uint32_t ch = wave_id; ch < n_chunks; ch += n_waves) {
@@ -328,60 +395,35 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
         records_read += fill;
         const Candidate *recs = a.cand + (size_t)ch * BFJ_CHUNK;
         const uint32_t n_groups = (fill + BFJ_WAVE - 1) / BFJ_WAVE; // at most 16
-        // Pass 1: count the certain-in pairs, note which records go to the refine list.
-        uint32_t band_groups = 0, n_band = 0;
-        auto count_body = [&](const Decided &d, const auto &s, uint32_t g) {
-            // A record with an eps in the band is decided whole by the FP64 refine;
+        // Count the certain-in pairs; write the records with an eps in the band.
+        auto body = [&](const Decided &d, const auto &s, uint32_t) {
+            // A record with an eps in the band is decided whole by the later stages;
             // otherwise it counts once, under the first eps that certainly holds it.
             const bool band = (d.band_i | d.band_j) != 0u;
             const int bi = (!band && d.in_i) ? __builtin_ctz(d.in_i) : -1;
             const int bj = (!band && d.in_j) ? __builtin_ctz(d.in_j) : -1;
             count_rows(a, tab, use_tab, s, d.i, bi, d.j, bj, lane);
             if (a.emit_pairs) append_pairs(a, a.sets[d.si], !band && (d.in_i & 1u) && d.i != d.j, d.i, d.j, lane);
-            const uint32_t nb = (uint32_t)__popcll(__ballot(band));
-            if (nb) band_groups |= 1u << g;
-            n_band += nb;
+            BandEntry be;
+            be.i = d.i;
+            be.j = d.j;
+            be.z = (float)d.c.z;
+            be.meta = (d.band_i | d.band_j) | ((d.c.col & BFJ_COL_SYM) ? 0x8000u : 0u) | ((uint32_t)d.si << 16);
+            band_append(a, bc, band, be, lane);
         };
         for (uint32_t g0 = 0; g0 < n_groups; g0 += G) {
             const uint32_t left = n_groups - g0;
-            classify_groups(a, recs, fill, g0, left >= (uint32_t)G ? (1u << G) - 1u : (1u << left) - 1u, lane,
-                            count_body);
-        }
-        if (!n_band) continue;
-        // One atomic per chunk reserves its refine-list records (a counter that every
-        // wavefront hits: per chunk rather than per 64 records keeps it uncontended).
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(a.band_count, (unsigned long long)n_band);
-        base = __shfl(base, 0);
-        // Pass 2: decide the band groups again and write their records.
-        auto band_body = [&](const Decided &d, const auto &, uint32_t) {
-            const bool band = (d.band_i | d.band_j) != 0u;
-            const unsigned long long bb = __ballot(band);
-            if (band) {
-                const unsigned long long idx = base + (unsigned long long)__popcll(bb & below);
-                if (idx < a.band_capacity) {
-                    BandEntry be;
-                    be.i = d.i;
-                    be.j = d.j;
-                    be.z = (float)d.c.z;
-                    be.meta = (d.band_i | d.band_j) | ((d.c.col & BFJ_COL_SYM) ? 0x8000u : 0u) | ((uint32_t)d.si << 16);
-                    a.band[idx] = be;
-                }
-            }
-            base += (unsigned long long)__popcll(bb);
-        };
-        for (uint32_t g0 = 0; g0 < n_groups; g0 += G) {
-            const uint32_t m = (band_groups >> g0) & ((1u << G) - 1u);
-            if (m) classify_groups(a, recs, fill, g0, m, lane, band_body);
+            classify_groups(a, recs, fill, g0, left >= (uint32_t)G ? (1u << G) - 1u : (1u << left) - 1u, lane, body);
         }
     }
+    band_close(a, bc, lane);
     if (use_tab) ctab_flush(a, tab, lane);
     if (lane == 0 && records_read) atomicAdd(a.cand_records, records_read);
 }
 
 void bfj_launch_classify(const PassArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_classify, dim3(2048), dim3(BFJ_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(k_classify, dim3(BFJ_CLASSIFY_WAVES / (BFJ_BLOCK / BFJ_WAVE)), dim3(BFJ_BLOCK), 0, st, a);
 }
 
 #define RK 8 // doubles per row per register chunk
@@ -499,7 +541,7 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_refine(PassArgs a)
             be.z = 0.f;
             be.meta = 0;
             if (live) be = a.band[t];
-            refine_one(a, be, live, lane, tab, use_tab);
+            refine_one(a, be, live && bfj_band_live(be), lane, tab, use_tab);
         }
     }
     if (use_tab) ctab_flush(a, tab, lane);
@@ -625,12 +667,12 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
         const uint64_t end = run + BFJ_REFINE_RUN < m ? run + BFJ_REFINE_RUN : m;
         for (uint64_t base = run; base < end; base += BFJ_WAVE) {
             const uint64_t t = base + lane;
-            const bool live = t < end;
             BandEntry be;
             be.i = be.j = 0;
             be.z = 0.f;
             be.meta = 0;
-            if (live) be = a.band[t];
+            if (t < end) be = a.band[t];
+            const bool live = t < end && bfj_band_live(be);
             const SetDesc &s = a.sets[be.meta >> 16];
             // S = ||c_i - c_j||^2 of the 64 records in 8 rounds of 8.  Each record's two
             // rows are read by a group of 8 lanes, lane h taking the row's 16-byte units h,
