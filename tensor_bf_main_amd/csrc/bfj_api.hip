@@ -198,7 +198,7 @@ struct bfj_ctx {
     int n_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[5] = {};
-    DevBuf pack, nq, gq, he, row_set, qerr, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
+    DevBuf pack, nq, gq, he, row_set, qerr, dnq, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
         outbuf, cand, chunk_fill, pairs, ref_x, ref_aux, perm, label, ohist, oseg, pivots, pivot_norm, mid_rows,
         mid_q, mid_band;
     int reorder = 1;   // pack whole-dataset passes in a locality order (order.hip)
@@ -498,6 +498,7 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->he, nrows * 16));
     RC(ensure(ctx->row_set, nrows * 2));
     RC(ensure(ctx->qerr, nrows * 8));
+    RC(ensure(ctx->dnq, nrows * 8));
     RC(ensure(ctx->nrm, nrows * 8));
     RC(ensure(ctx->scales, nrows + 1024));
     RC(ensure(ctx->cnt, std::max<int64_t>(p.cnt_total, 1) * 4));
@@ -576,6 +577,7 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.he = (double2 *)ctx->he.p;
     a.row_set = (uint16_t *)ctx->row_set.p;
     a.qerr = (double *)ctx->qerr.p;
+    a.dnq = (double *)ctx->dnq.p;
     a.nrm = (double *)ctx->nrm.p;
     a.scales = (uint8_t *)ctx->scales.p;
     a.cnt = (uint32_t *)ctx->cnt.p;
@@ -822,7 +824,7 @@ __global__ void k_synth(double *x, int64_t total, uint64_t seed)
 // Every device buffer of a context.
 static std::vector<DevBuf *> all_bufs(bfj_ctx *c)
 {
-    return {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->nrm, &c->scales, &c->cnt,
+    return {&c->pack, &c->nq, &c->gq, &c->he, &c->row_set, &c->qerr, &c->dnq, &c->nrm, &c->scales, &c->cnt,
             &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
             &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux,
             &c->perm, &c->label, &c->ohist, &c->oseg, &c->pivots, &c->pivot_norm, &c->mid_rows,

@@ -171,6 +171,7 @@ struct PassArgs {
     double2 *he;               // per row
     uint16_t *row_set;         // per row: index of its dataset
     double *qerr, *nrm;        // per row
+    double *dnq;               // per row: n_j minus the join's candidate threshold for it (FP8; else 0)
     uint8_t *scales;
     uint32_t *cnt;
     int64_t cnt_len;

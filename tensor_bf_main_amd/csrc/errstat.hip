@@ -47,10 +47,10 @@ __device__ inline double direct_d2(const double *p, const double *q, int d)
 // Does d2 (input units) lie in the interval the band tests take for the pair: the
 // exact distance D = dist(y_i, y_j) (kernel space) within [sqrt(lo) - r, sqrt(hi) + r]
 // for S in [lo, hi], and d2 = sigma^2 D^2 (1 +- rho)?  (The same bounds as k_classify.)
-__device__ inline bool in_certified(double d2, double sig2, double shat, double E, double z, double hx, double r,
-                                    int d)
+__device__ inline bool in_certified(double d2, double sig2, double shat, double E, double z, double hx, double dj,
+                                    double r, int d)
 {
-    const double mg = (fabs(hx) + 2.0 * fabs(z) + E) * 0x1p-50;
+    const double mg = (fabs(hx) + 2.0 * fabs(z) + 2.0 * dj + E) * 0x1p-50;
     const double lo = shat - E - mg, hi = shat + E + mg;
     const double rho = 1.01 * (d + 2) * 0x1p-53;
     const double Dlo = sqrt(d2 / sig2 / (1.0 + rho)), Dhi = sqrt(d2 / sig2 / (1.0 - rho));
@@ -83,10 +83,11 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_errcand(PassArgs a, ErrOut *out, 
             const double *q = s.x + (int64_t)bfj_row_at(a, s, col - s.row_off) * s.ld;
             const double d2 = direct_d2(p, q, s.d);
             const double2 h = a.he[c.row];
-            const double shat = h.x - 2.0 * c.z;
+            const double dj = a.dnq[col]; // the record holds Z' = acc - n'_j = Z + dnq_j
+            const double shat = h.x - 2.0 * c.z + 2.0 * dj;
             emax = fmax(emax, fabs(sig2 * shat - d2));
             const double r = (a.qerr[c.row] + a.qerr[col]) * (1.0 + 0x1p-50);
-            bad += in_certified(d2, sig2, shat, h.y, c.z, h.x, r, s.d) ? 0 : 1;
+            bad += in_certified(d2, sig2, shat, h.y, c.z, h.x, dj, r, s.d) ? 0 : 1;
             ++checked;
         }
     }
@@ -174,13 +175,14 @@ __global__ __launch_bounds__(BFJ_WAVE) void k_errtile(PassArgs a, const ErrTile 
             if (pi >= s.n || pj >= s.n) continue;
             const uint32_t ri = bfj_row_at(a, s, pi), rj = bfj_row_at(a, s, pj);
             if ((int64_t)ri < s.row_begin || (int64_t)ri >= s.row_end) continue; // not a query row: no g_i
-            const double z = (double)(acc[r] - nj); // Z as the join forms it (in val_t)
+            const double z = (double)(acc[r] - nj); // Z' as the join forms it (in val_t)
             const double2 h = a.he[s.row_off + pi];
-            const double shat = h.x - 2.0 * z;
+            const double dj = a.dnq[s.row_off + pj];
+            const double shat = h.x - 2.0 * z + 2.0 * dj;
             const double d2 = direct_d2(s.x + (int64_t)ri * s.ld, s.x + (int64_t)rj * s.ld, s.d);
             emax = fmax(emax, fabs(sig2 * shat - d2));
             const double qi = a.qerr[s.row_off + pi], qj = a.qerr[s.row_off + pj];
-            bad += in_certified(d2, sig2, shat, h.y, z, h.x, (qi + qj) * (1.0 + 0x1p-50), s.d) ? 0 : 1;
+            bad += in_certified(d2, sig2, shat, h.y, z, h.x, dj, (qi + qj) * (1.0 + 0x1p-50), s.d) ? 0 : 1;
             ++checked;
             if (LOW) {
                 double sq = 0.0; // exact up to FP64 rounding: differences of FP32-exact values

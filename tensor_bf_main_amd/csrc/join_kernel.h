@@ -18,6 +18,9 @@
 //     d2_approx(i, j)): the squared norms and the threshold are folded in with
 //     no VALU work before the MFMAs, and the test Z >= 0 is acc >= n_j (the
 //     reference pre-loads candidate chunk norms, kernel_join.cu:433-437).
+//     For FP8, n_j is the candidate threshold k_thresholds writes (prep.hip): half
+//     the squared norm lowered by the row's share of the quantisation margin, so
+//     that a row with a small cast error rejects more.
 //   * Z < 0 means "certainly out" for every epsilon.  One max tree and one wave
 //     vote skip almost every tile.  The pairs of a hot tile with Z >= 0 are
 //     appended to a candidate stream (row, col, Z) with wave-compacted plain

@@ -30,7 +30,9 @@
 //   with r = q_i + q_j (one bound for both rows of the pair).  g_i = 0.5 (Hmax_i +
 //   E_i - N_i) rounded up, Hmax_i the largest H of r = q_i + q_max (>= H(r_ij) for
 //   every j), makes Z < 0 certainly outside for every eps and both rows: one Z per
-//   unordered pair decides for i and for j.
+//   unordered pair decides for i and for j.  FP8 passes split Hmax into a row part
+//   (in g_i) and a column part (in the join's candidate threshold for n_j);
+//   k_thresholds has the details.
 #include "bfj_internal.h"
 #include <math.h>
 #include <float.h>
@@ -533,7 +535,7 @@ __global__ void k_thresholds(PassArgs a)
         // Hmax: the largest banded H of this row (r = q_i + q_max).  Every pair (i, j)
         // is decided with its own r_ij = q_i + q_j <= q_i + q_max, so H(r_ij) <= Hmax
         // and Z < 0 is a certain reject of the pair, for both of its rows.
-        double Hmax = -INFINITY;
+        double Hmax = -INFINITY, shi_max = -INFINITY;
         for (int e = 0; e < s.n_eps; ++e) {
             const double T = s.T[e] / sigma2; // kernel space (exact power-of-two scaling)
             Hc[e] = Lc[e] = -INFINITY;
@@ -548,9 +550,38 @@ __global__ void k_thresholds(PassArgs a)
                 Hc[e] = (shi + r) * (shi + r) * (1.0 + 0x1p-49);
                 if (query) halfband = fmax(halfband, 0.5 * (Hc[e] - (Lc[e] > 0.0 ? Lc[e] : 0.0)) * sigma2);
                 Hmax = fmax(Hmax, Hc[e]);
+                shi_max = fmax(shi_max, shi);
             }
         }
         if (i == 0 && all_mask) atomicOr(&a.scal[si].all_mask, (unsigned long long)all_mask);
+        // FP8 passes split the reject threshold into a row part and a column part: for
+        // every banded eps and every partner j,
+        //   (shi_e + q_i + q_j)^2 <= (shi_max + q_i)^2 + [2 (shi_max + q_max) q_j + q_j^2]
+        // (q_i <= q_max; both parts rounded up, with room for the (1 + 2^-49) and the
+        // (1 + 2^-50) of the band tests).  g_i carries the row part Arow; the join's
+        // candidate threshold n'_j = n_j - Bcol_j / 2 (rounded down) carries the column
+        // part, and dnq_j = n_j - n'_j lets the band tests rebuild Z = acc - n_j.  A pair
+        // whose q_j is below q_max is then rejected by the join instead of streamed
+        // (FP8 workload 4: 40 % fewer candidates).  The other dtypes keep Hmax and
+        // n'_j = n_j: their radii are small against the thresholds, so the split
+        // removes a few per cent of the candidates, less than the extra gather of dnq
+        // per record costs k_classify.
+        const bool split = a.dtype == BFJ_FP8 && Hmax != -INFINITY;
+        const double up = 1.0 + 0x1p-46;
+        const double qi_u = a.qerr[row] * up, qm_u = qmax * up;
+        const double Arow = split ? (shi_max + qi_u) * (shi_max + qi_u) * up : Hmax;
+        const double Bcol = split ? (2.0 * (shi_max + qm_u) * qi_u + qi_u * qi_u) * up : 0.0;
+        const double Bmax = split ? (2.0 * (shi_max + qm_u) * qm_u + qm_u * qm_u) * up : 0.0;
+        double dn = 0.0;
+        if (split && i < s.n) {
+            const double nqv = (double)((float *)a.nq)[row];
+            double t = nqv - 0.5 * Bcol;
+            t -= fabs(t) * 0x1p-50 + 0x1p-1000; // below the exact difference (4x its rounding)
+            const float nqn = f32_down(t);
+            ((float *)a.nq)[row] = nqn;
+            dn = nqv - (double)nqn;
+        }
+        a.dnq[row] = dn;
         double g; // FP64 value of the per-row initial term actually stored
         double E = 0.0;
         if (!query || Hmax == -INFINITY) {
@@ -562,15 +593,19 @@ __global__ void k_thresholds(PassArgs a)
             const double Kp = 2.0 * K + 8.0;
             const double coef = 2.0 * (1.01 * Kp + 1.0) * u;
             const double cross = sqrt(Ni * nmax);
-            double gabs = 0.5 * fabs(Hmax - Ni);
+            double gabs = 0.5 * fabs(Arow - Ni);
             for (int it = 0; it < 2; ++it) {
-                E = coef * (gabs + cross) + 4.0 * u * nmax + 4.0 * (K + 4) * u64 * (Ni + nmax + gabs) + K * 0x1p-120;
-                gabs = 0.5 * fabs(Hmax - Ni) + E;
+                // 4 u Bmax: the final subtraction acc - n'_j, whose operand is up to
+                // Bmax / 2 further from acc than n_j.
+                E = coef * (gabs + cross) + 4.0 * u * (nmax + Bmax) + 4.0 * (K + 4) * u64 * (Ni + nmax + gabs) +
+                    K * 0x1p-120;
+                gabs = 0.5 * fabs(Arow - Ni) + E;
             }
             E *= 1.01;
-            // Rounded up whatever its sign: 2 g + N_i - E >= Hmax makes Z < 0 a certain
-            // reject (S >= S_hat - E > Hmax >= H(r_ij)) for every banded eps.
-            const double x = 0.5 * (Hmax + E - Ni);
+            // Rounded up whatever its sign: 2 g + N_i - E >= Arow, so Z' = acc - n'_j < 0
+            // gives S_hat = 2 g + N_i - 2 (Z' - dnq_j) > Arow + E + Bcol_j, a certain reject
+            // (S >= S_hat - E > Arow + Bcol_j >= H(r_ij)) for every banded eps.
+            const double x = 0.5 * (Arow + E - Ni);
             const double gv = x + fabs(x) * 0x1p-50 + 0x1p-1000;
             g = fp64 ? gv : (double)f32_up(gv);
         }

@@ -229,10 +229,15 @@ struct Fetched {
     Candidate c;
     double2 h;     // he[row]
     double qi, qj; // quantisation radii of both rows
+    double dj;     // dnq[col]: n_j minus the join's candidate threshold
     int si;        // dataset
     bool live;
 };
 
+// SPLIT: the pass lowered the join's candidate thresholds (FP8, k_thresholds), so the
+// record's Z' needs dnq of its column; a separate instance keeps the other passes'
+// registers (and occupancy) as they were.
+template <bool SPLIT>
 __device__ inline void fetch_record(const PassArgs &a, const Candidate *recs, uint32_t t, uint32_t fill, Fetched &f)
 {
     f.live = t < fill;
@@ -241,6 +246,7 @@ __device__ inline void fetch_record(const PassArgs &a, const Candidate *recs, ui
     f.h = a.he[f.c.row];
     f.qi = a.qerr[f.c.row];
     f.qj = a.qerr[f.c.col & ~BFJ_COL_SYM];
+    f.dj = SPLIT ? a.dnq[f.c.col & ~BFJ_COL_SYM] : 0.0;
 }
 
 // s: the record's dataset (row_off); b: its bounds (slo, shi); n_eps, band_mask of it.
@@ -255,10 +261,11 @@ __device__ inline Decided decide_fetched(const Fetched &f, const Set &s, const B
     o.j = (f.c.col & ~BFJ_COL_SYM) - (uint32_t)s.row_off;
     o.in_i = o.band_i = o.in_j = o.band_j = 0;
     if (f.live) {
-        // S_hat = 2 g_i + N_i - 2 Z, |S_hat - S| <= E_i; mg covers the FP64 rounding
-        // of these few operations.
-        const double shat = f.h.x - 2.0 * f.c.z;
-        const double mg = (fabs(f.h.x) + 2.0 * fabs(f.c.z) + f.h.y) * 0x1p-50;
+        // S_hat = 2 g_i + N_i - 2 Z with Z = acc - n_j = Z' - dnq_j (the record holds
+        // Z' = acc - n'_j), |S_hat - S| <= E_i; mg covers the FP64 rounding of these
+        // few operations.
+        const double shat = f.h.x - 2.0 * f.c.z + 2.0 * f.dj;
+        const double mg = (fabs(f.h.x) + 2.0 * fabs(f.c.z) + 2.0 * f.dj + f.h.y) * 0x1p-50;
         const double lo = shat - f.h.y - mg, hi = shat + f.h.y + mg;
         // |dist(xhat_i, xhat_j) - dist(y_i, y_j)| <= q_i + q_j: the same bounds hold
         // for both rows of the pair.
@@ -344,14 +351,14 @@ __device__ inline void band_close(const PassArgs &a, const BandCursor &bc, int l
 // of gmask (uniform).  When every record of these groups is from one dataset (the
 // common case: a chunk is filled by one join wavefront), its descriptor and scalars
 // are read once through scalar loads; otherwise per lane.  All lanes take part.
-template <class Body>
+template <bool SPLIT, class Body>
 __device__ inline void classify_groups(const PassArgs &a, const Candidate *recs, uint32_t fill, uint32_t g0,
                                        uint32_t gmask, int lane, Body &&body)
 {
     constexpr int G = BFJ_CLS_GROUPS;
     Fetched f[G];
 #pragma unroll
-    for (int k = 0; k < G; ++k) fetch_record(a, recs, (g0 + k) * BFJ_WAVE + lane, fill, f[k]);
+    for (int k = 0; k < G; ++k) fetch_record<SPLIT>(a, recs, (g0 + k) * BFJ_WAVE + lane, fill, f[k]);
     const int si0 = __builtin_amdgcn_readfirstlane(f[0].si);
     bool same = true;
 #pragma unroll
@@ -373,6 +380,7 @@ __device__ inline void classify_groups(const PassArgs &a, const Candidate *recs,
     }
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
 {
     constexpr int G = BFJ_CLS_GROUPS;
@@ -413,7 +421,8 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
         };
         for (uint32_t g0 = 0; g0 < n_groups; g0 += G) {
             const uint32_t left = n_groups - g0;
-            classify_groups(a, recs, fill, g0, left >= (uint32_t)G ? (1u << G) - 1u : (1u << left) - 1u, lane, body);
+            classify_groups<SPLIT>(a, recs, fill, g0, left >= (uint32_t)G ? (1u << G) - 1u : (1u << left) - 1u, lane,
+                                   body);
         }
     }
     band_close(a, bc, lane);
@@ -423,7 +432,9 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_classify(PassArgs a)
 
 void bfj_launch_classify(const PassArgs &a, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_classify, dim3(BFJ_CLASSIFY_WAVES / (BFJ_BLOCK / BFJ_WAVE)), dim3(BFJ_BLOCK), 0, st, a);
+    const dim3 grid(BFJ_CLASSIFY_WAVES / (BFJ_BLOCK / BFJ_WAVE));
+    if (a.dtype == BFJ_FP8) hipLaunchKernelGGL(k_classify<true>, grid, dim3(BFJ_BLOCK), 0, st, a);
+    else hipLaunchKernelGGL(k_classify<false>, grid, dim3(BFJ_BLOCK), 0, st, a);
 }
 
 #define RK 8 // doubles per row per register chunk
