@@ -144,6 +144,25 @@ static int lds_for(int dtype, int kc, int rows, int n_eps)
 //  * FP16/BF16 with 1-2 K-chunks (d <= 64): a candidate block is too little MFMA work
 //    to cover its epilogue, so resident wavefronts matter more than strip width
 //    (n = 1M: R = 64 beats R = 128 by 8 % at d = 16 and d = 64).
+// Work items of one K-chunk group for a panel of `panel` candidate blocks (the loop
+// that builds them, counted only).
+static int64_t count_items(const std::vector<SetDesc> &sets, int kc, int rows, int panel)
+{
+    const int strip = BFJ_JWAVES * rows / 16;
+    int64_t n = 0;
+    for (const SetDesc &s : sets) {
+        if (s.kc != kc || s.row_end <= s.row_begin) continue;
+        const int nblk = (int)(s.npad / 16);
+        const int rb_begin = (int)(s.row_begin / 16), rb_end = (int)((s.row_end + 15) / 16);
+        for (int c0 = 0; c0 < nblk; c0 += panel) {
+            const int c1 = std::min(nblk, c0 + panel);
+            for (int rb = rb_begin; rb < rb_end; rb += strip)
+                if ((s.sym ? std::max(c0, rb & ~3) : c0) < c1) ++n;
+        }
+    }
+    return n;
+}
+
 static int heuristic_rows(int dtype, int kc, int n_eps, bool reordered)
 {
     const bool low = dtype == BFJ_FP16 || dtype == BFJ_BF16;
@@ -341,6 +360,23 @@ static int validate(const HostSet &h, int dtype)
     return BFJ_OK;
 }
 
+// Panel bytes for a group with no panel size set: BFJ_PANEL_BYTES, halved (down to
+// 32 candidate blocks per item) while the group has fewer than two items per block of
+// its persistent grid.  A small dataset otherwise runs a few long items on part of the
+// chip (n = 10,000, FP64: 175 items of up to 256 blocks for 512 resident blocks).
+static int small_pass_panel(const bfj_ctx *ctx, const PassPlan &p, const Group &g, int FB)
+{
+    const int64_t tile_bytes = (int64_t)g.kc * BFJ_WAVE * FB;
+    int occ = bfj_join_max_blocks_per_cu(p.dtype, g.kc, g.rows, g.lds);
+    if (g.cap > 0) occ = std::min(occ, g.cap);
+    const int64_t want = 2ll * std::max(1, occ) * ctx->n_cu;
+    int64_t bytes = BFJ_PANEL_BYTES;
+    while (bytes / tile_bytes / 16 * 16 > 32 &&
+           count_items(p.sets, g.kc, g.rows, (int)std::max<int64_t>(32, bytes / tile_bytes / 16 * 16)) < want)
+        bytes /= 2;
+    return (int)bytes;
+}
+
 static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, PassPlan &p, int force_rows)
 {
     p = PassPlan();
@@ -424,6 +460,7 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
         g.lds = lds_for(dtype, kc, rows, n_eps_max);
         g.cap = ctx->tile_cap[dtype][kc];
         g.panel_bytes = ctx->tile_panel[dtype][kc] ? ctx->tile_panel[dtype][kc] : BFJ_PANEL_BYTES;
+        if (!ctx->tile_panel[dtype][kc]) g.panel_bytes = small_pass_panel(ctx, p, g, FB);
         groups.push_back(g);
         key.insert(key.end(), {kc, rows, g.lds, g.cap, g.panel_bytes});
     }
