@@ -134,16 +134,6 @@ static int lds_for(int dtype, int kc, int rows, int n_eps)
     return 0;
 }
 
-// Default tile rows R: the widest strip among those with the most resident blocks per
-// CU, counted up to a cap (latency hiding across blocks matters more than strip width
-// up to there).  Measured on MI355X:
-//  * FP8 (twice the MFMA rate, so more issue and LDS work per flop) wants 4 blocks per
-//    CU, FP16/BF16/FP64 3 in input order (R = 64 at d = 128);
-//  * in the locality order hot tiles are rare and 2 blocks of wider strips win
-//    (FP16 d = 128: R = 128, 76 ms against 78 ms at R = 64);
-//  * FP16/BF16 with 1-2 K-chunks (d <= 64): a candidate block is too little MFMA work
-//    to cover its epilogue, so resident wavefronts matter more than strip width
-//    (n = 1M: R = 64 beats R = 128 by 8 % at d = 16 and d = 64).
 // Work items of one K-chunk group for a panel of `panel` candidate blocks (the loop
 // that builds them, counted only).
 static int64_t count_items(const std::vector<SetDesc> &sets, int kc, int rows, int panel)
@@ -163,6 +153,16 @@ static int64_t count_items(const std::vector<SetDesc> &sets, int kc, int rows, i
     return n;
 }
 
+// Default tile rows R: the widest strip among those with the most resident blocks per
+// CU, counted up to a cap (latency hiding across blocks matters more than strip width
+// up to there).  Measured on MI355X:
+//  * FP8 (twice the MFMA rate, so more issue and LDS work per flop) wants 4 blocks per
+//    CU, FP16/BF16/FP64 3 in input order (R = 64 at d = 128);
+//  * in the locality order hot tiles are rare and 2 blocks of wider strips win
+//    (FP16 d = 128: R = 128, 76 ms against 78 ms at R = 64);
+//  * FP16/BF16 with 1-2 K-chunks (d <= 64): a candidate block is too little MFMA work
+//    to cover its epilogue, so resident wavefronts matter more than strip width
+//    (n = 1M: R = 64 beats R = 128 by 8 % at d = 16 and d = 64).
 static int heuristic_rows(int dtype, int kc, int n_eps, bool reordered)
 {
     const bool low = dtype == BFJ_FP16 || dtype == BFJ_BF16;
@@ -741,14 +741,14 @@ static int check_finite(const std::vector<SetScalars> &sc)
 // past the capacity were dropped), grow both to what the pass asked for and set
 // *again so that the caller repeats the pass.
 #define BFJ_MAX_CHUNKS (1ull << 26)   // 64 Mi chunks of 1024 candidate records (1 TiB): more is refused
-#define BFJ_MAX_BAND (1ull << 34)     // band-list records (256 GiB per list): more is refused
+#define BFJ_MAX_BAND (1ull << 34)     // band-list slots (256 GiB per list): more is refused
 static int grow_on_overflow(bfj_ctx *ctx, const unsigned long long *ctr, bool *again)
 {
     *again = false;
     const unsigned long long band_count = ctr[BFJ_CTR_BAND], chunks = ctr[BFJ_CTR_CHUNKS];
     if (ctx->noalloc && (chunks > ctx->chunk_capacity || band_count > ctx->band_capacity))
         return fail(BFJ_ERR_CAPACITY,
-                    "the pass needs %llu candidate chunks and %llu band records; the no-allocation scratch "
+                    "the pass needs %llu candidate chunks and %llu band-list slots; the no-allocation scratch "
                     "(%zu bytes) holds %u and %llu for this dataset",
                     chunks, band_count, BFJ_SCRATCH_BYTES, ctx->chunk_capacity,
                     (unsigned long long)ctx->band_capacity);
@@ -763,9 +763,15 @@ static int grow_on_overflow(bfj_ctx *ctx, const unsigned long long *ctr, bool *a
     }
     if (band_count > ctx->band_capacity) {
         if (band_count > BFJ_MAX_BAND)
-            return fail(BFJ_ERR_UNSUPPORTED, "the pass puts %llu pairs in the band (at most %llu)", band_count,
+            return fail(BFJ_ERR_UNSUPPORTED, "the pass needs %llu band-list slots (at most %llu)", band_count,
                         BFJ_MAX_BAND);
-        ctx->band_capacity = std::min<unsigned long long>(band_count + band_count / 4 + 1024, BFJ_MAX_BAND);
+        // band_count counts reserved slots: the records plus the holes of classify's
+        // last blocks.  The larger capacity may give larger blocks (pass_args); while
+        // a block is above its 64-slot minimum the holes stay within a quarter of the
+        // capacity, so 1.5x the slots holds the same pass again (records + capacity / 4
+        // <= slots + capacity / 4 <= capacity).  At the minimum the holes are at most
+        // 8192 x 64 slots, which the growth covers within a repeat or two more.
+        ctx->band_capacity = std::min<unsigned long long>(band_count + band_count / 2 + 1024, BFJ_MAX_BAND);
         RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
         if (ctx->mid_band.p) RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
         *again = true;
@@ -839,7 +845,7 @@ static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>
             return BFJ_OK;
         }
         if (++retries > 4)
-            return fail(BFJ_ERR_INTERNAL, "candidate stream / refine list keep overflowing (%llu chunks, %llu records)",
+            return fail(BFJ_ERR_INTERNAL, "candidate stream / band list keep overflowing (%llu chunks, %llu slots)",
                         ctr[BFJ_CTR_CHUNKS], ctr[BFJ_CTR_BAND]);
         RC(stage_plan(ctx, p)); // the grown buffers may have moved
     }
@@ -1483,7 +1489,7 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
         RC(stage_plan(ctx, pl->pass));
         RC(capture(pl));
     }
-    return fail(BFJ_ERR_INTERNAL, "candidate stream / refine list keep overflowing");
+    return fail(BFJ_ERR_INTERNAL, "candidate stream / band list keep overflowing");
 }
 
 int bfj_plan_tiles(const bfj_plan *pl, int32_t *kc, int32_t *rows, int32_t *blocks_per_cu, int32_t *panel_bytes,
