@@ -1087,8 +1087,8 @@ int bfj_self_join_device_noalloc(bfj_ctx *ctx, const double *d_data, int64_t n, 
     if (n > 0) {
         // Sizing pass (no device work): the buffers every pass of this shape needs, with
         // the smallest candidate stream and band lists; the rest of the scratch then goes
-        // to those two, 60 % to the candidate stream (FP8 bands hold about 0.6 records
-        // per candidate, in two lists).
+        // to those two: 60 % to the candidate stream, or a third for FP8, whose
+        // candidates are nearly all band records (each kept in two lists of 16 bytes).
         HostSet h;
         memset(&h, 0, sizeof h);
         h.x = d_data;
@@ -1112,8 +1112,9 @@ int bfj_self_join_device_noalloc(bfj_ctx *ctx, const double *d_data, int64_t n, 
         const size_t fixed = arena.used + 64 * 1024; // + alignment of every buffer
         const size_t rest = BFJ_SCRATCH_BYTES > fixed ? BFJ_SCRATCH_BYTES - fixed : 0;
         const size_t lists = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? 2 : 1;
-        const size_t chunks = rest / 10 * 6 / ((size_t)BFJ_CHUNK * sizeof(Candidate) + 4);
-        const size_t band = rest / 10 * 4 / (lists * sizeof(BandEntry));
+        const size_t cand_share = p.dtype == BFJ_FP8 ? rest / 3 : rest / 10 * 6;
+        const size_t chunks = cand_share / ((size_t)BFJ_CHUNK * sizeof(Candidate) + 4);
+        const size_t band = (rest - cand_share) / (lists * sizeof(BandEntry));
         if (chunks < 64 || band < 1024)
             return fail(BFJ_ERR_CAPACITY, "n = %lld, d = %d leaves too little of the %zu-byte no-allocation scratch "
                         "for the candidate stream", (long long)n, d, BFJ_SCRATCH_BYTES);
