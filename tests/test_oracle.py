@@ -198,3 +198,25 @@ def test_parse_text_matches_reference_reader():
             np.testing.assert_array_equal(ref, mine)
         finally:
             os.unlink(path)
+
+
+def test_thread_count_and_counts_independent_of_threads():
+    """num_threads reports OpenMP's thread count (OMP_NUM_THREADS in a fresh process),
+    and the counts do not depend on how many threads share the rows."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", "from oracle import oracle; print(oracle.num_threads())"],
+                       cwd=root, env=dict(os.environ, OMP_NUM_THREADS="3"), capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "3", r.stderr[-2000:]
+    assert oracle.num_threads() >= 1
+    rng = np.random.default_rng(11)
+    x = rng.random((700, 9))
+    eps = [0.35, 0.5]
+    c1, t1 = oracle.join(x, eps, nthreads=1)
+    c5, t5 = oracle.join(x, eps, nthreads=5)
+    assert np.array_equal(c1, c5) and np.array_equal(t1, t5)
+    rows = rng.choice(700, size=50, replace=False)
+    assert np.array_equal(oracle.join_rows(x, rows, eps, nthreads=1), oracle.join_rows(x, rows, eps, nthreads=7))
+    assert np.array_equal(oracle.join_rows(x, rows, eps, nthreads=3), c1[:, rows])
