@@ -66,6 +66,21 @@ def roofline(flops, join_ms, dtype, traffic=None):
             "kernel": "k_join (MFMA join + fused epsilon epilogue)"}
 
 
+def host_cpu():
+    """CPU model, logical cores of the host and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"model": model, "logical_cores": os.cpu_count(), "usable_cores": usable}
+
+
 def cpu_baseline(x, eps, rows_n, got_counts, seed=4242):
     """Oracle on a bounded sample of query rows; returns (baseline dict, mismatches)."""
     from oracle import oracle
@@ -80,7 +95,7 @@ def cpu_baseline(x, eps, rows_n, got_counts, seed=4242):
     what = f"all {n} rows (full run)" if rows_n == n else f"{rows_n} seeded query rows"
     return {"value": rows_n * float(n) / tc, "unit": "pair-distances/s", "cores": oracle.num_threads(),
             "kind": "port", "sample": f"{what} x all {n} candidates, FP64 direct form "
-                                      f"(OpenMP), {tc:.3g} s"}, mism
+                                      f"(OpenMP), {tc:.3g} s", "host": host_cpu()}, mism
 
 
 def pass_errors(ctx, max_candidates=1 << 24, sample_tiles=4096):
