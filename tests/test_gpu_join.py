@@ -212,7 +212,7 @@ def test_unsorted_eps_lists(ctx, dtype, sym):
         _check(ctx, x, eps, dtype, row_begin=700, row_end=2100)
 
 
-@pytest.mark.parametrize("dtype", ["fp16", "fp64"])
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_batch_mixed_dims(ctx, dtype):
     sets = wl.batch(n_sets=12, seed=77, n_min=100, n_max=3000)
     out, stats = ctx.self_join_batch(sets, dtype=dtype)
