@@ -100,23 +100,24 @@ __device__ inline void cand_close(const JoinArgs &a, const CandCursor &cc, int l
 }
 
 // Append the pairs of one 16x16 output block with acc >= n_j (Z >= 0) as candidate
-// records, wave-compacted: one ballot per register, at most one chunk grab, then
-// plain 16-byte stores.  In a symmetric sweep only pairs above the diagonal
-// (col > row) are kept: the tile that holds (j, i) is dropped by the caller and
-// self pairs are counted in k_finalize.
+// records, wave-compacted: one lane mask per output register (a single compare each),
+// at most one chunk grab, then one 16-byte store per pair at the chunk's uniform base
+// plus a 32-bit lane offset.  The caller has set the accumulators of the pairs not to
+// stream to -inf (in a symmetric sweep only pairs above the diagonal, col > row, are
+// kept, since the tile that holds (j, i) is dropped and self pairs are counted in
+// k_finalize).  row0 is the global row of the block's first row, colrec the record's
+// column word.
 template <class Tr>
 __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &cc, const typename Tr::acc_t &v,
-                                                typename Tr::val_t nj, int col, bool col_ok, int li0, bool sym,
-                                                uint32_t row0, uint32_t row_off, int lane)
+                                                typename Tr::val_t nj, uint32_t row0, uint32_t colrec, int lane)
 {
     unsigned long long bal[4];
-    uint32_t cnt[4], nrec = 0;
+    uint32_t off[4], nrec = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int li = li0 + Tr::out_row(lane, r);
-        bal[r] = __builtin_amdgcn_ballot_w64(v[r] >= nj && col_ok && (!sym || col > li));
-        cnt[r] = (uint32_t)__popcll(bal[r]);
-        nrec += cnt[r];
+        bal[r] = __builtin_amdgcn_ballot_w64(v[r] >= nj);
+        off[r] = nrec;
+        nrec += (uint32_t)__popcll(bal[r]);
     }
     if (nrec == 0) return;
     if (cc.fill + nrec > BFJ_CHUNK) { // nrec <= 64 fits a fresh chunk
@@ -132,20 +133,18 @@ __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &c
         cc.fill = 0;
     }
     if (cc.chunk < a.chunk_capacity) {
-        const unsigned long long below = (1ull << lane) - 1ull;
-        Candidate *dst = a.cand + (size_t)cc.chunk * BFJ_CHUNK + cc.fill;
-        const uint32_t tag = sym ? BFJ_COL_SYM : 0u;
-        uint32_t off = 0;
+        Candidate *dst = a.cand + (size_t)cc.chunk * BFJ_CHUNK + cc.fill; // uniform
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            if ((bal[r] >> lane) & 1ull) {
+            if (v[r] >= nj) { // this lane's bit of bal[r]
+                const uint32_t k = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(bal[r] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal[r], off[r]));
                 Candidate rec;
                 rec.row = row0 + (uint32_t)Tr::out_row(lane, r);
-                rec.col = (row_off + (uint32_t)col) | tag;
+                rec.col = colrec;
                 rec.z = (double)(v[r] - nj);
-                dst[off + (uint32_t)__popcll(bal[r] & below)] = rec;
+                dst[k] = rec;
             }
-            off += cnt[r];
         }
     }
     cc.fill += nrec;
@@ -307,8 +306,9 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
                 const int lim = cb - wrb0;
                 hot &= lim < 0 ? 0u : (lim >= RB - 1 ? ~0u : (2u << lim) - 1u);
             }
-            const int col = cb * 16 + (lane & 15);
-            const bool col_ok = col < s.n;
+            // Padding columns need no mask: their n_j is 1e30 (prep), beyond any
+            // acc = g_i + xhat_i . 0.
+            const uint32_t colrec = (uint32_t)(s.row_off + cb * 16 + (lane & 15)) | (sym ? BFJ_COL_SYM : 0u);
             while (hot) {
                 const int rb = __builtin_ctz(hot);
                 hot &= hot - 1u;
@@ -317,8 +317,12 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 #pragma unroll
                 for (int k = 1; k < RB; ++k)
                     if (rb == k) v = acc[k];
-                cand_push_block<Tr>(a, cc, v, nj, col, col_ok, blk * 16, sym, (uint32_t)(s.row_off + blk * 16),
-                                    (uint32_t)s.row_off, lane);
+                if (sym && blk == cb) { // the diagonal block: only col > row
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        if ((lane & 15) <= Tr::out_row(lane, r)) v[r] = -(val_t)__builtin_huge_val();
+                }
+                cand_push_block<Tr>(a, cc, v, nj, (uint32_t)(s.row_off + blk * 16), colrec, lane);
             }
         }
     };
