@@ -99,9 +99,9 @@ def test_host_rows_keeps_views_and_copies_the_rest():
 
 
 def test_tile_options_and_errors():
-    assert bfj.tile_rows_options("fp16", 128) == [32, 64, 96, 128]
+    assert bfj.tile_rows_options("fp16", 128) == [16, 32, 64, 96, 128]
     assert bfj.tile_rows_options("fp64", 128) == [16, 32]
-    assert bfj.tile_rows_options("fp8", 256) == [32, 64, 96, 128]
+    assert bfj.tile_rows_options("fp8", 256) == [16, 32, 64, 96, 128]
     assert bfj.tile_rows_options("bf16", 200) == [32, 64]
     for d in (1, 7, 31, 33, 64, 65, 96, 129, 200, 255, 256):
         for dt in bfj.DTYPES:

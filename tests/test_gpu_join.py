@@ -458,6 +458,29 @@ def test_autotune_then_exact(ctx):
     ctx.set_tile_rows("fp16", 0)
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("d", [16, 96, 200, 256])
+def test_every_built_tile_shape_is_exact(ctx, dtype, d):
+    """Every strip width built for (dtype, d), forced in turn, on the symmetric sweep and on
+    a row range: the oracle's counts.  Clustered rows and eps up to the 2 % quantile make hot
+    tiles common, and n = 1507 leaves a ragged last block and diagonal blocks in every strip."""
+    x = wl.gmm(1507, d, 8, 0.05, 31 + d)
+    eps = wl.eps_quantile(x, [0.002, 0.02], 300, 5)
+    oc, ot = oracle.join(x, eps)
+    oc_r, ot_r = oracle.join(x, eps, 100, 1200)
+    try:
+        for r in bfj.tile_rows_options(dtype, d):
+            ctx.set_tile_rows(dtype, r)
+            res = ctx.self_join(x, eps, dtype=dtype)
+            np.testing.assert_array_equal(res.counts, oc, err_msg=f"{dtype} d={d} R={r}")
+            np.testing.assert_array_equal(res.totals, ot, err_msg=f"{dtype} d={d} R={r}")
+            res = ctx.self_join(x, eps, dtype=dtype, row_begin=100, row_end=1200)
+            np.testing.assert_array_equal(res.counts, oc_r, err_msg=f"{dtype} d={d} R={r} rows 100-1200")
+            np.testing.assert_array_equal(res.totals, ot_r, err_msg=f"{dtype} d={d} R={r} rows 100-1200")
+    finally:
+        ctx.set_tile_rows(dtype, 0)
+
+
 @pytest.mark.slow
 def test_workload3_full_fp16_vs_fp64(ctx):
     """n = 1M, d = 128: FP16+refine and FP64 counts agree exactly; 1000 seeded rows
