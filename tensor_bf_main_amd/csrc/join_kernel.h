@@ -90,8 +90,10 @@ struct JoinLds {
 // are handed out by one atomic each; the records themselves are plain stores
 // that nothing waits on.
 struct CandCursor {
+    Candidate *dst; // next slot of the current chunk (uniform)
     uint32_t chunk; // current chunk (0xffffffff: none yet)
     uint32_t fill;  // records written into it
+    bool live;      // the chunk lies within the capacity (else its records are dropped)
 };
 
 __device__ inline void cand_close(const JoinArgs &a, const CandCursor &cc, int lane)
@@ -120,7 +122,7 @@ __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &c
         nrec += (uint32_t)__popcll(bal[r]);
     }
     if (nrec == 0) return;
-    if (cc.fill + nrec > BFJ_CHUNK) { // nrec <= 64 fits a fresh chunk
+    if (cc.fill + nrec > BFJ_CHUNK) { // nrec <= 4 x 64 fits a fresh chunk
         cand_close(a, cc, lane);
         // 64-bit counter, saturated to a 32-bit index: once past the capacity every
         // later chunk is dropped, never written over an earlier one.
@@ -131,9 +133,11 @@ __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &c
         }
         cc.chunk = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl(c, 0));
         cc.fill = 0;
+        cc.live = cc.chunk < a.chunk_capacity;
+        cc.dst = a.cand + (size_t)(cc.live ? cc.chunk : 0u) * BFJ_CHUNK;
     }
-    if (cc.chunk < a.chunk_capacity) {
-        Candidate *dst = a.cand + (size_t)cc.chunk * BFJ_CHUNK + cc.fill; // uniform
+    if (cc.live) {
+        Candidate *dst = cc.dst;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             if (v[r] >= nj) { // this lane's bit of bal[r]
@@ -147,6 +151,7 @@ __device__ __forceinline__ void cand_push_block(const JoinArgs &a, CandCursor &c
             }
         }
     }
+    cc.dst += nrec;
     cc.fill += nrec;
 }
 
@@ -302,9 +307,11 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             uint32_t hot = 0;
 #pragma unroll
             for (int rb = 0; rb < RB; ++rb) hot |= (uint32_t)(__builtin_amdgcn_ballot_w64(mrb[rb] >= nj) != 0ull) << rb;
-            if (sym) { // keep the row blocks rb <= cb - wrb0
+            // In a symmetric sweep keep the row blocks rb <= cb - wrb0: all of them once
+            // the candidate block has passed this wavefront's last row block.
+            if (sym && cb < wrb0 + RB - 1) {
                 const int lim = cb - wrb0;
-                hot &= lim < 0 ? 0u : (lim >= RB - 1 ? ~0u : (2u << lim) - 1u);
+                hot &= lim < 0 ? 0u : (2u << lim) - 1u;
             }
             // Padding columns need no mask: their n_j is 1e30 (prep), beyond any
             // acc = g_i + xhat_i . 0.
@@ -317,11 +324,12 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
 #pragma unroll
                 for (int k = 1; k < RB; ++k)
                     if (rb == k) v = acc[k];
-                if (sym && blk == cb) { // the diagonal block: only col > row
+                // The diagonal block keeps only col > row: one lane mask per register
+                // (fixed per lane) and'ed with a uniform flag.
+                const bool diag = sym && blk == cb;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        if ((lane & 15) <= Tr::out_row(lane, r)) v[r] = -(val_t)__builtin_huge_val();
-                }
+                for (int r = 0; r < 4; ++r)
+                    v[r] = (diag && (lane & 15) <= Tr::out_row(lane, r)) ? -(val_t)__builtin_huge_val() : v[r];
                 cand_push_block<Tr>(a, cc, v, nj, (uint32_t)(s.row_off + blk * 16), colrec, lane);
             }
         }
@@ -432,8 +440,10 @@ __global__ __launch_bounds__(BFJ_JBLOCK) BFJ_JOIN_ATTR void k_join(JoinArgs a)
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     __shared__ uint32_t item_s;
     CandCursor cc;
+    cc.dst = a.cand;
     cc.chunk = 0xffffffffu;
     cc.fill = BFJ_CHUNK;
+    cc.live = false;
     const int lane = bfj_lane();
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(a.item_counter, 1u);
