@@ -81,17 +81,39 @@ __device__ inline double wave_max(double v)
 // [row block][BFJ_MAXDIM] arrays: sums, maxima, minima) for the scale.
 __global__ void k_colsum(PassArgs a)
 {
+    // Threads as [row lane][column]: cw columns (d rounded up to a power of two) by
+    // BFJ_BLOCK / cw row lanes, so narrow rows keep every thread busy (d = 16: 16 rows
+    // at a time instead of 16 threads of 256).  Row lane rr sums rows rr, rr + rl, ...
+    // in order, and the row lanes' partials are then added in lane order: a fixed
+    // order, so the sums are the same bits on every run.
+    static_assert(BFJ_MAXDIM <= BFJ_BLOCK, "one thread per column at the widest rows");
+    __shared__ double red[3][BFJ_BLOCK];
     const int b = blockIdx.x;
     const SetDesc &s = a.sets[a.blk_set[b]];
     const int64_t r0 = a.blk_row0[b];
     const int64_t plane = (int64_t)a.n_row_blocks * BFJ_MAXDIM;
-    for (int k = threadIdx.x; k < s.d; k += blockDim.x) {
-        double acc = 0.0, mx = -INFINITY, mn = INFINITY;
-        for (int64_t i = r0; i < r0 + BFJ_PREP_ROWS && i < s.n; ++i) {
+    int cw = 1;
+    while (cw < s.d) cw <<= 1; // d <= BFJ_MAXDIM = BFJ_BLOCK
+    const int rl = BFJ_BLOCK / cw;
+    const int k = threadIdx.x % cw, rr = threadIdx.x / cw;
+    double acc = 0.0, mx = -INFINITY, mn = INFINITY;
+    if (k < s.d) {
+        for (int64_t i = r0 + rr; i < r0 + BFJ_PREP_ROWS && i < s.n; i += rl) {
             const double v = s.x[i * s.ld + k];
             acc += v;
             mx = fmax(mx, v);
             mn = fmin(mn, v);
+        }
+    }
+    red[0][threadIdx.x] = acc;
+    red[1][threadIdx.x] = mx;
+    red[2][threadIdx.x] = mn;
+    __syncthreads();
+    if (rr == 0 && k < s.d) {
+        for (int q = 1; q < rl; ++q) {
+            acc += red[0][q * cw + k];
+            mx = fmax(mx, red[1][q * cw + k]);
+            mn = fmin(mn, red[2][q * cw + k]);
         }
         a.colsum[(int64_t)b * BFJ_MAXDIM + k] = acc;
         a.colsum[plane + (int64_t)b * BFJ_MAXDIM + k] = mx;
