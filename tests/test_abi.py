@@ -181,3 +181,29 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_profile_traffic_lines_parse():
+    """bench.py's roofline.traffic comes from the profiles' machine-readable lines: each one
+    parses to a positive byte count, and the headline configuration (workload 3, FP16,
+    R = 128) and the FP64 workload 2 resolve to theirs."""
+    import importlib.util
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    prof = os.path.join(root, "profiles")
+    lines = [ln.strip() for name in sorted(os.listdir(prof)) if name.endswith(".md")
+             for ln in open(os.path.join(prof, name)) if ln.startswith("machine-readable:")]
+    assert lines
+    pat = re.compile(r"^machine-readable: workload=(\d) dtype=(\w+) tile_rows=(\d+) traffic_bytes=(\d+)$")
+    for ln in lines:
+        m = pat.match(ln)
+        assert m, ln
+        w, dt, r, b = int(m.group(1)), m.group(2), int(m.group(3)), int(m.group(4))
+        assert b > 0
+        assert bench.measured_traffic(w, dt, r) == float(b)
+    assert bench.measured_traffic(3, "fp16", 128) > 1e9
+    assert bench.measured_traffic(2, "fp64", 16) > 1e8
+    assert bench.measured_traffic(3, "fp16", 64) is None
