@@ -661,11 +661,91 @@ __device__ inline double mid_chunk(const uint4 (&p)[4], const uint4 (&q)[4], dou
     return S;
 }
 
+// The middle stage of 64 band records, one per lane (`live`: the lane holds one):
+// S = ||c_i - c_j||^2 from the stage's copies, then decide, count or forward to the
+// FP64 refine.  The records of lanes src < lim are read (holes among them too: a
+// hole reads row 0 and is discarded).  All lanes take part.
+template <bool F32>
+__device__ inline void mid_records(const PassArgs &a, CountTable &tab, bool use_tab, const BandEntry &be, bool live,
+                                   int lim, int lane)
+{
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const SetDesc &s = a.sets[be.meta >> 16];
+    // S = ||c_i - c_j||^2 of the 64 records in 8 rounds of 8.  Each record's two
+    // rows are read by a group of 8 lanes, lane h taking the row's 16-byte units h,
+    // h + 8, ..., so one load covers 128 contiguous bytes of each of 8 rows (one
+    // lane per record would touch 64 cache lines per load).  The group's partial
+    // sums meet by butterfly and go back to the record's own lane.  k_mid_pack and
+    // k_quantize16 wrote the columns past d up to a whole unit count as zeros.
+    constexpr int ES = F32 ? 4 : 2; // bytes per value of the copy
+    const int grp = lane >> 3, h = lane & 7;
+    double S = 0.0;
+    for (int b = 0; b < 8; ++b) {
+        const int src = 8 * b + grp;
+        const uint32_t ri = __shfl(be.i, src), rj = __shfl(be.j, src), rm = __shfl(be.meta, src);
+        double Sg = 0.0;
+        if (src < lim) {
+            const SetDesc &rs = a.sets[rm >> 16];
+            const int units = F32 ? (rs.d + 15) / 16 * 4 : (rs.d + 31) / 32 * 4;
+            const uint4 *p = reinterpret_cast<const uint4 *>(a.mid_rows + (rs.row_off + ri) * BFJ_MAXDIM * ES);
+            const uint4 *q = reinterpret_cast<const uint4 *>(a.mid_rows + (rs.row_off + rj) * BFJ_MAXDIM * ES);
+            for (int u0 = h; u0 < units; u0 += 32) {
+                // Unconditional loads (the index clamped to the row's last unit)
+                // so that all eight are in flight together; units past the row
+                // are then zeroed.
+                uint4 pp[4], qq[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int u = u0 + 8 * k;
+                    const int uc = u < units ? u : units - 1;
+                    pp[k] = p[uc];
+                    qq[k] = q[uc];
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (u0 + 8 * k >= units) pp[k] = qq[k] = make_uint4(0u, 0u, 0u, 0u);
+                Sg = mid_chunk<F32>(pp, qq, Sg);
+            }
+        }
+        Sg += __shfl_xor(Sg, 1);
+        Sg += __shfl_xor(Sg, 2);
+        Sg += __shfl_xor(Sg, 4);
+        const double Sb = __shfl(Sg, 8 * h);
+        if (grp == b) S = Sb;
+    }
+    uint32_t in = 0, band = 0;
+    if (live) {
+        const SetScalars &sc = a.scal[be.meta >> 16];
+        const double r = (a.mid_q[s.row_off + be.i] + a.mid_q[s.row_off + be.j]) * (1.0 + 0x1p-50);
+        decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, S * (1.0 - 0x1p-40), S * (1.0 + 0x1p-40), in, band);
+    }
+    // Still in a band: decided whole by the FP64 refine.  Otherwise the record
+    // counts once, under the first eps that certainly holds it (if any).
+    const bool fwd = live && band != 0u;
+    const int k = (live && !fwd && in) ? __builtin_ctz(in) : -1;
+    count_rows(a, tab, use_tab, s, be.i, k, be.j, (be.meta & 0x8000u) ? k : -1, lane);
+    if (a.emit_pairs) append_pairs(a, s, k == 0 && be.i != be.j, be.i, be.j, lane);
+    const unsigned long long fb = __ballot(fwd);
+    if (fb) {
+        unsigned long long b0 = 0;
+        if (lane == 0) b0 = atomicAdd(a.mid_count, (unsigned long long)__popcll(fb));
+        b0 = __shfl(b0, 0);
+        if (fwd) {
+            const unsigned long long idx = b0 + (unsigned long long)__popcll(fb & below);
+            if (idx < a.band_capacity) a.mid_band[idx] = be;
+        }
+    }
+}
+
+// Records a wavefront stages and regroups by query row at a time, in the FP16-copy
+// (FP8 pass) middle stage (a multiple of 64).
+#ifndef BFJ_MID_SORT
+#define BFJ_MID_SORT 256
+#endif
 template <bool F32>
 __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
 {
     const int lane = threadIdx.x & 63;
-    const unsigned long long below = (1ull << lane) - 1ull;
     const unsigned long long cnt = *a.band_count;
     const uint64_t m = cnt < a.band_capacity ? cnt : a.band_capacity;
     const uint64_t wave = blockIdx.x * (BFJ_BLOCK / BFJ_WAVE) + (threadIdx.x >> 6);
@@ -674,80 +754,94 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_mid(PassArgs a)
     CountTable &tab = tabs[threadIdx.x >> 6];
     const bool use_tab = a.cnt_len < (int64_t)BFJ_CTAB_EMPTY;
     if (use_tab) ctab_clear(tab, lane);
-    for (uint64_t run = wave * BFJ_REFINE_RUN; run < m; run += step) {
-        const uint64_t end = run + BFJ_REFINE_RUN < m ? run + BFJ_REFINE_RUN : m;
-        for (uint64_t base = run; base < end; base += BFJ_WAVE) {
-            const uint64_t t = base + lane;
-            BandEntry be;
-            be.i = be.j = 0;
-            be.z = 0.f;
-            be.meta = 0;
-            if (t < end) be = a.band[t];
-            const bool live = t < end && bfj_band_live(be);
-            const SetDesc &s = a.sets[be.meta >> 16];
-            // S = ||c_i - c_j||^2 of the 64 records in 8 rounds of 8.  Each record's two
-            // rows are read by a group of 8 lanes, lane h taking the row's 16-byte units h,
-            // h + 8, ..., so one load covers 128 contiguous bytes of each of 8 rows (one
-            // lane per record would touch 64 cache lines per load).  The group's partial
-            // sums meet by butterfly and go back to the record's own lane.  k_mid_pack and
-            // k_quantize16 wrote the columns past d up to a whole unit count as zeros.
-            constexpr int ES = F32 ? 4 : 2; // bytes per value of the copy
-            const int grp = lane >> 3, h = lane & 7;
-            double S = 0.0;
-            for (int b = 0; b < 8; ++b) {
-                const int src = 8 * b + grp;
-                const uint32_t ri = __shfl(be.i, src), rj = __shfl(be.j, src), rm = __shfl(be.meta, src);
-                double Sg = 0.0;
-                if (base + src < end) {
-                    const SetDesc &rs = a.sets[rm >> 16];
-                    const int units = F32 ? (rs.d + 15) / 16 * 4 : (rs.d + 31) / 32 * 4;
-                    const uint4 *p = reinterpret_cast<const uint4 *>(a.mid_rows + (rs.row_off + ri) * BFJ_MAXDIM * ES);
-                    const uint4 *q = reinterpret_cast<const uint4 *>(a.mid_rows + (rs.row_off + rj) * BFJ_MAXDIM * ES);
-                    for (int u0 = h; u0 < units; u0 += 32) {
-                        // Unconditional loads (the index clamped to the row's last unit)
-                        // so that all eight are in flight together; units past the row
-                        // are then zeroed.
-                        uint4 pp[4], qq[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const int u = u0 + 8 * k;
-                            const int uc = u < units ? u : units - 1;
-                            pp[k] = p[uc];
-                            qq[k] = q[uc];
-                        }
-#pragma unroll
-                        for (int k = 0; k < 4; ++k)
-                            if (u0 + 8 * k >= units) pp[k] = qq[k] = make_uint4(0u, 0u, 0u, 0u);
-                        Sg = mid_chunk<F32>(pp, qq, Sg);
-                    }
-                }
-                Sg += __shfl_xor(Sg, 1);
-                Sg += __shfl_xor(Sg, 2);
-                Sg += __shfl_xor(Sg, 4);
-                const double Sb = __shfl(Sg, 8 * h);
-                if (grp == b) S = Sb;
+    if constexpr (F32) {
+        // FP32 copies (FP16 and BF16 passes): the records in stream order, 64 at a time.
+        for (uint64_t run = wave * BFJ_REFINE_RUN; run < m; run += step) {
+            const uint64_t end = run + BFJ_REFINE_RUN < m ? run + BFJ_REFINE_RUN : m;
+            for (uint64_t base = run; base < end; base += BFJ_WAVE) {
+                const uint64_t t = base + lane;
+                BandEntry be;
+                be.i = be.j = 0;
+                be.z = 0.f;
+                be.meta = 0;
+                if (t < end) be = a.band[t];
+                const bool live = t < end && bfj_band_live(be);
+                mid_records<F32>(a, tab, use_tab, be, live, (int)(end - base < BFJ_WAVE ? end - base : BFJ_WAVE),
+                                 lane);
             }
-            uint32_t in = 0, band = 0;
-            if (live) {
-                const SetScalars &sc = a.scal[be.meta >> 16];
-                const double r = (a.mid_q[s.row_off + be.i] + a.mid_q[s.row_off + be.j]) * (1.0 + 0x1p-50);
-                decide(sc, (uint32_t)sc.band_mask, s.n_eps, r, S * (1.0 - 0x1p-40), S * (1.0 + 0x1p-40), in, band);
-            }
-            // Still in a band: decided whole by the FP64 refine.  Otherwise the record
-            // counts once, under the first eps that certainly holds it (if any).
-            const bool fwd = live && band != 0u;
-            const int k = (live && !fwd && in) ? __builtin_ctz(in) : -1;
-            count_rows(a, tab, use_tab, s, be.i, k, be.j, (be.meta & 0x8000u) ? k : -1, lane);
-            if (a.emit_pairs) append_pairs(a, s, k == 0 && be.i != be.j, be.i, be.j, lane);
-            const unsigned long long fb = __ballot(fwd);
-            if (fb) {
-                unsigned long long b0 = 0;
-                if (lane == 0) b0 = atomicAdd(a.mid_count, (unsigned long long)__popcll(fb));
-                b0 = __shfl(b0, 0);
-                if (fwd) {
-                    const unsigned long long idx = b0 + (unsigned long long)__popcll(fb & below);
-                    if (idx < a.band_capacity) a.mid_band[idx] = be;
+        }
+    } else {
+        // FP16 copies (FP8 passes): the live records of each BFJ_MID_SORT-record window are
+        // first regrouped in LDS by the low 6 bits of their query row i (a counting sort:
+        // one LDS atomic per record gives its rank in its bucket, a wave scan the bucket
+        // offsets); holes are dropped.  The band records of an FP8 pass are sparse over
+        // the strips' rows, so in stream order the 8 records of a round read mostly
+        // different query rows; the records of one join chunk come from a strip of R <= 64
+        // consecutive rows, so after the regrouping a bucket holds one row and the 8
+        // records of a round mostly share row i, whose loads are then the same addresses in
+        // one instruction, fetched once.
+        constexpr int PER = BFJ_MID_SORT / BFJ_WAVE; // records per lane while regrouping
+        __shared__ BandEntry srt[BFJ_BLOCK / BFJ_WAVE][BFJ_MID_SORT];
+        __shared__ uint32_t bkt[BFJ_BLOCK / BFJ_WAVE][BFJ_WAVE];
+        BandEntry *mys = srt[threadIdx.x >> 6];
+        uint32_t *myb = bkt[threadIdx.x >> 6];
+        for (uint64_t run = wave * BFJ_REFINE_RUN; run < m; run += step) {
+            const uint64_t end = run + BFJ_REFINE_RUN < m ? run + BFJ_REFINE_RUN : m;
+            for (uint64_t win = run; win < end; win += BFJ_MID_SORT) {
+                myb[lane] = 0;
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                BandEntry rr[PER];
+                uint32_t rank[PER];
+                int key[PER];
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const uint64_t t = win + (uint64_t)(k * BFJ_WAVE + lane);
+                    rr[k].i = rr[k].j = 0;
+                    rr[k].z = 0.f;
+                    rr[k].meta = 0;
+                    if (t < end) rr[k] = a.band[t];
+                    const bool lv = t < end && bfj_band_live(rr[k]);
+                    key[k] = lv ? (int)(rr[k].i & (BFJ_WAVE - 1)) : -1;
+                    rank[k] = lv ? __hip_atomic_fetch_add(&myb[key[k]], 1u, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP)
+                                 : 0u;
                 }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                // Exclusive scan of the 64 bucket counts; nl = live records of the window.
+                const uint32_t cnt_b = myb[lane];
+                uint32_t incl = cnt_b;
+#pragma unroll
+                for (int o = 1; o < BFJ_WAVE; o <<= 1) {
+                    const uint32_t v = __shfl_up(incl, o);
+                    if (lane >= o) incl += v;
+                }
+                const uint32_t nl = (uint32_t)__shfl((int)incl, BFJ_WAVE - 1);
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                myb[lane] = incl - cnt_b;
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                // Bucket offset + rank < nl <= BFJ_MID_SORT.
+#pragma unroll
+                for (int k = 0; k < PER; ++k)
+                    if (key[k] >= 0) mys[myb[key[k]] + rank[k]] = rr[k];
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t base = 0; base < nl; base += BFJ_WAVE) {
+                    const bool live = base + lane < nl;
+                    BandEntry be;
+                    be.i = be.j = 0;
+                    be.z = 0.f;
+                    be.meta = 0;
+                    if (live) be = mys[base + lane];
+                    mid_records<F32>(a, tab, use_tab, be, live, (int)(nl - base < BFJ_WAVE ? nl - base : BFJ_WAVE),
+                                     lane);
+                }
+                // The staging and the counters are rewritten only after these reads.
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
