@@ -147,6 +147,7 @@ def single_dataset(args, ctx, x, eps, dtype, metric, workload, fp64_counts=None)
         "band_fraction": last.stats["band_pairs"] / pairs,
         "fp64_refine_pairs": int(last.stats["fp64_pairs"]),
         "candidates": int(last.stats["candidates"]),
+        "slices": int(last.stats["slices"]),
         "max_band_halfwidth": last.stats["max_band_halfwidth"],
         "stage_ms": {k: last.stats[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
         # Kernel-only time of a pass (prep + join + classify/refine/finalize, HIP events),
@@ -301,7 +302,7 @@ def batch_workload(args, ctx):
            "per_dataset_launches_ms": t_each * 1e3, "per_dataset_pairs_per_s": pairs / t_each,
            "graph_speedup_vs_per_dataset": t_each / t_graph, "plan_create_s": t_plan,
            "band_pairs": int(st["band_pairs"]), "fp64_refine_pairs": int(st["fp64_pairs"]),
-           "candidates": int(st["candidates"]),
+           "candidates": int(st["candidates"]), "slices": int(st["slices"]),
            "stage_ms": {k: st[k] for k in ("ms_prep", "ms_join", "ms_refine", "ms_total")},
            "roofline": roofline(flops, st["ms_join"], args.dtype), "errors": errors, "tiles": plan.tiles(),
            "mismatches_graph_vs_eager": mismatches}

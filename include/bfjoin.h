@@ -37,9 +37,10 @@ extern "C" {
 
 /* 2: bfj_stats gained fp64_pairs.  3: bfj_stats lost max_abs_d2_err (see bfj_pass_errors)
    and gained candidates; bfj_set_capacity, bfj_fail_allocation, bfj_pass_errors,
-   bfj_plan_tiles and bfj_self_join_device_noalloc.  (Another struct layout: callers built against an
-   earlier version must be rebuilt.) */
-#define BFJ_ABI_VERSION 3
+   bfj_plan_tiles and bfj_self_join_device_noalloc.  4: bfj_stats gained slices;
+   bfj_set_stream_limit.  (Another struct layout: callers built against an earlier
+   version must be rebuilt.) */
+#define BFJ_ABI_VERSION 4
 
 typedef struct bfj_ctx bfj_ctx;
 typedef struct bfj_plan bfj_plan;
@@ -82,6 +83,9 @@ typedef struct bfj_stats {
                                 passes) left undecided; band_pairs for FP64 passes */
     uint64_t candidates;     /* pair records the join streamed to the band tests (the pairs it
                                 could not reject) */
+    int32_t slices;          /* pieces the join's work ran in (1: one pass; more when the
+                                candidate stream exceeded the stream limit) */
+    int32_t reserved;
 } bfj_stats;
 
 /* One dataset of a batch.  data/counts are host or device pointers depending on
@@ -146,9 +150,10 @@ int bfj_self_join_device(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t 
  * counters) come from one file-scope device array of BFJ_NOALLOC_SCRATCH_BYTES that
  * each GPU holds until the process ends, not from hipMalloc.  Limits: the dataset
  * must fit that array (about 1.5 KiB per row at d = 128 FP16 besides the candidate
- * stream and band lists, which take the rest: n up to roughly 16k rows), and a pass
- * whose candidate stream or band list does not fit returns BFJ_ERR_CAPACITY instead
- * of growing them.  Calls on one GPU share its array, so the library runs them one
+ * stream and band lists, which take the rest: n up to roughly 16k rows; a larger
+ * dataset returns BFJ_ERR_CAPACITY).  A pass whose candidate stream or band list
+ * does not fit runs in slices within the array (bfj_set_stream_limit) instead of
+ * growing them.  Calls on one GPU share its array, so the library runs them one
  * at a time (a per-device lock); bfj_pass_errors does not apply to them. */
 #define BFJ_NOALLOC_SCRATCH_BYTES (64ull << 20)
 int bfj_self_join_device_noalloc(bfj_ctx *ctx, const double *d_data, int64_t n, int32_t d, int64_t ld,
@@ -227,6 +232,17 @@ int bfj_set_reorder(bfj_ctx *ctx, int32_t enable);
  * for a known workload, or to force the grow-and-repeat path in tests. */
 int bfj_set_capacity(bfj_ctx *ctx, uint64_t candidate_chunks, uint64_t band_records);
 
+/* Device memory the candidate stream and the band lists of a pass may take, in bytes
+ * (0, the default: half of the device's memory).  No counterpart in the reference,
+ * which streams no candidates.  A pass whose stream would need more runs the join's
+ * work items in slices, each followed by its band tests, middle stage and FP64
+ * refine; the counts are the same (stats slices > 1).  A no-allocation call slices
+ * within its fixed buffers the same way.  Sliced passes synchronise with the host
+ * after every slice and are not captured: a plan that needed slices runs them
+ * eagerly from then on.  A limit too small for one work item's candidates fails with
+ * BFJ_ERR_CAPACITY. */
+int bfj_set_stream_limit(bfj_ctx *ctx, uint64_t bytes);
+
 /* Fault injection for tests of the error paths (no counterpart in the reference):
  * the countdown-th device allocation the library makes from now on (over every context
  * of the process) fails as if the device were out of memory, and the call that made it
@@ -242,6 +258,7 @@ int bfj_fail_allocation(int64_t countdown);
  *   (b) sample_tiles seeded 16 x 16 tiles of all pairs, spread over the datasets by
  *       their pair counts and recomputed with the pass's own MFMA sequence.
  * Every checked pair is also tested against the interval the band tests assumed.
+ * After a pass that ran in slices, (a) covers the records of its last slice.
  * Runs after the pass, outside its timing; it needs the pass's buffers and rows,
  * so call it before any other call on the context and before freeing the rows. */
 int bfj_pass_errors(bfj_ctx *ctx, uint64_t max_candidates, int64_t sample_tiles, uint64_t seed, bfj_errors *out);

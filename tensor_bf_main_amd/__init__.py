@@ -55,6 +55,8 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double),
         ("fp64_pairs", ctypes.c_uint64),
         ("candidates", ctypes.c_uint64),
+        ("slices", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -92,7 +94,7 @@ class Dataset(ctypes.Structure):
 
 
 _lib = None
-ABI_VERSION = 3  # BFJ_ABI_VERSION of include/bfjoin.h that Stats and the prototypes mirror
+ABI_VERSION = 4  # BFJ_ABI_VERSION of include/bfjoin.h that Stats and the prototypes mirror
 
 
 def load_library(path: str = LIB_PATH):
@@ -129,6 +131,7 @@ def load_library(path: str = LIB_PATH):
         "bfj_set_tile_rows": (ctypes.c_int, [vp, i32, i32]),
         "bfj_set_reorder": (ctypes.c_int, [vp, i32]),
         "bfj_set_capacity": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint64]),
+        "bfj_set_stream_limit": (ctypes.c_int, [vp, ctypes.c_uint64]),
         "bfj_fail_allocation": (ctypes.c_int, [i64]),
         "bfj_pass_errors": (ctypes.c_int, [vp, ctypes.c_uint64, i64, ctypes.c_uint64, ctypes.POINTER(Errors)]),
         "bfj_tile_rows_options": (ctypes.c_int, [i32, i32, ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]),
@@ -425,6 +428,12 @@ class Context:
         the next pass (0 leaves one as it is).  A pass that needs more grows them and is
         repeated, so results never depend on these."""
         _check(self.lib.bfj_set_capacity(self._h, int(candidate_chunks), int(band_records)))
+
+    def set_stream_limit(self, nbytes: int = 0):
+        """Device memory the candidate stream and band lists of a pass may take (0: half of
+        the device's memory).  A pass that would need more runs the join in slices, each
+        followed by its band tests; the counts are the same (stats ``slices`` > 1)."""
+        _check(self.lib.bfj_set_stream_limit(self._h, int(nbytes)))
 
     def pass_errors(self, max_candidates: int = 1 << 24, sample_tiles: int = 4096, seed: int = 12345) -> dict:
         """Accuracy of the last pass on this context (call it right after the pass): max

@@ -226,6 +226,8 @@ struct bfj_ctx {
     uint64_t pair_capacity = 0; // pair-list mode (bfj_self_join_pairs) only
     int emit_pairs = 0;
     uint32_t chunk_capacity = 0; // candidate chunks of BFJ_CHUNK records
+    uint64_t stream_limit = 0;   // bytes the candidate stream and band lists may take (0: half of mem_total)
+    uint64_t mem_total = 0;      // the device's memory
     int tile_rows[4][33] = {};  // [dtype][kc] forced/tuned R (0 = heuristic)
     int tile_cap[4][33] = {};   // [dtype][kc] tuned resident blocks per CU (0 = as many as fit)
     int tile_panel[4][33] = {}; // [dtype][kc] tuned panel bytes (0 = BFJ_PANEL_BYTES)
@@ -339,6 +341,8 @@ struct PassPlan {
     int64_t rows_total = 0, pack_bytes = 0, cnt_total = 0;
     uint64_t pairs = 0;
     int max_rows = 0, max_kc = 0;
+    int sliced = 0;                     // the candidate stream does not fit: the join runs in slices
+    unsigned long long stream_chunks = 0; // candidate chunks of the last sliced run (sizes the next one's slices)
 };
 
 static int validate(const HostSet &h, int dtype)
@@ -521,6 +525,32 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
     return BFJ_OK;
 }
 
+// Bytes the candidate stream and the band lists of a pass may take.
+static uint64_t stream_limit(const bfj_ctx *ctx)
+{
+    return ctx->stream_limit ? ctx->stream_limit : ctx->mem_total / 2;
+}
+
+// Band lists of a pass: the band list, and the middle stage's list when it runs.
+static uint64_t band_lists(const bfj_ctx *ctx, const PassPlan &p)
+{
+    return (p.dtype != BFJ_FP64 && ctx->mid_stage) ? 2 : 1;
+}
+
+// Capacities of the candidate stream and the band lists cut in proportion to fit the
+// stream limit (a pass that needs more then overflows and runs in slices).
+static void clamp_to_limit(bfj_ctx *ctx, const PassPlan &p)
+{
+    if (ctx->noalloc) return;
+    const double bytes = (double)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate) +
+                         (double)ctx->band_capacity * sizeof(BandEntry) * (double)band_lists(ctx, p);
+    const double limit = (double)stream_limit(ctx);
+    if (bytes <= limit) return;
+    const double f = limit / bytes;
+    ctx->chunk_capacity = (uint32_t)std::max(1.0, (double)ctx->chunk_capacity * f);
+    ctx->band_capacity = (uint64_t)std::max(1024.0, (double)ctx->band_capacity * f);
+}
+
 // Buffers sized for the plan; descriptors uploaded.
 static int stage_plan(bfj_ctx *ctx, PassPlan &p)
 {
@@ -541,18 +571,19 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     RC(ensure(ctx->cnt, std::max<int64_t>(p.cnt_total, 1) * 4));
     if (ctx->band_capacity == 0)
         ctx->band_capacity = std::min<uint64_t>(std::max<uint64_t>((uint64_t)p.rows_total * 64, 1u << 20), 1u << 28);
-    RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
-    if (p.dtype != BFJ_FP64 && ctx->mid_stage) {
-        RC(ensure(ctx->mid_rows, nrows * BFJ_MAXDIM * (p.dtype == BFJ_FP8 ? 2 : 4)));
-        RC(ensure(ctx->mid_q, nrows * 8));
-        RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
-    }
     if (ctx->chunk_capacity == 0) {
         // Start at 96 candidates per row (1.5 GiB at a million rows; about 100 neighbours
         // per point fit) and at least 4 chunks per join wavefront of the GPU; a pass
         // that needs more is repeated with the size it reported.
         const uint64_t want = std::max<uint64_t>((uint64_t)p.rows_total * 96 / BFJ_CHUNK, 16384);
         ctx->chunk_capacity = (uint32_t)std::min<uint64_t>(want, 1u << 24);
+    }
+    clamp_to_limit(ctx, p);
+    RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+    if (p.dtype != BFJ_FP64 && ctx->mid_stage) {
+        RC(ensure(ctx->mid_rows, nrows * BFJ_MAXDIM * (p.dtype == BFJ_FP8 ? 2 : 4)));
+        RC(ensure(ctx->mid_q, nrows * 8));
+        RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
     }
     RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
     RC(ensure(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
@@ -591,6 +622,17 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     return BFJ_OK;
 }
 
+// k_classify's wavefronts reserve band slots in blocks; a block as large as a quarter
+// of the capacity spread over all of them (64..1024 slots) bounds the holes its partly
+// filled last blocks leave.
+static uint32_t band_block_for(uint64_t band_capacity)
+{
+    const uint64_t share = band_capacity / (4ull * BFJ_CLASSIFY_WAVES);
+    uint32_t blk = 64;
+    while (blk < 1024 && (uint64_t)blk * 2 <= share) blk *= 2;
+    return blk;
+}
+
 static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
 {
     PassArgs a;
@@ -624,15 +666,7 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     a.band_count = ctr + BFJ_CTR_BAND;
     a.band_capacity = ctx->band_capacity;
     a.band_records = ctr + BFJ_CTR_BANDREC;
-    // k_classify's wavefronts reserve band slots in blocks; a block as large as a
-    // quarter of the capacity spread over all of them (64..1024 slots) bounds the
-    // holes its partly filled last blocks leave.
-    {
-        const uint64_t share = ctx->band_capacity / (4ull * BFJ_CLASSIFY_WAVES);
-        uint32_t blk = 64;
-        while (blk < 1024 && (uint64_t)blk * 2 <= share) blk *= 2;
-        a.band_block = blk;
-    }
+    a.band_block = band_block_for(ctx->band_capacity);
     a.mid = (p.dtype != BFJ_FP64 && ctx->mid_stage) ? 1 : 0;
     a.mid_fp32 = p.dtype != BFJ_FP8 ? 1 : 0;
     a.mid_rows = (unsigned char *)ctx->mid_rows.p;
@@ -660,28 +694,33 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     return a;
 }
 
-// Enqueue the whole pass on the context stream (no host synchronisation, no
-// memset nodes: safe to capture in a graph).
-static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
+// The pass's counts, scalars and counters zeroed, then the prep kernels.
+static int enqueue_prep(bfj_ctx *ctx, const PassPlan &p, const PassArgs &a)
 {
-    ctx->last.reset();
-    const PassArgs a = pass_args(ctx, p);
     hipStream_t st = ctx->stream;
-    if (timed) HC(hipEventRecord(ctx->ev[0], st));
     bfj_launch_zero(ctx->cnt.p, (size_t)std::max<int64_t>(p.cnt_total, 1) * 4, st);
     bfj_launch_zero(ctx->scal.p, p.sets.size() * sizeof(SetScalars), st);
     bfj_launch_zero(ctx->counters.p, BFJ_CTR_SLOTS * 8, st);
     if (p.blk_set.size()) bfj_launch_prep(a, st);
     HC(hipGetLastError());
-    if (timed) HC(hipEventRecord(ctx->ev[1], st));
+    return BFJ_OK;
+}
+
+// The join over the work items [lo, hi) of the pass's list: one launch per K-chunk
+// group the range meets, each with its own work-queue head (zeroed beforehand).
+static int enqueue_join(bfj_ctx *ctx, const PassPlan &p, const PassArgs &a, size_t lo, size_t hi)
+{
+    hipStream_t st = ctx->stream;
     uint32_t *item_ctr = (uint32_t *)((unsigned long long *)ctx->counters.p + BFJ_CTR_ITEMS);
     for (size_t gi = 0; gi < p.groups.size(); ++gi) {
         const Group &g = p.groups[gi];
+        const size_t b = std::max(lo, (size_t)g.item0), e = std::min(hi, (size_t)g.item0 + (size_t)g.n_items);
+        if (b >= e) continue;
         JoinArgs j;
         memset(&j, 0, sizeof j);
         j.sets = a.sets;
-        j.items = (const WorkItem *)ctx->items.p + g.item0;
-        j.n_items = g.n_items;
+        j.items = (const WorkItem *)ctx->items.p + b;
+        j.n_items = (int32_t)(e - b);
         j.item_counter = item_ctr + gi % 32;
         j.pack = a.pack;
         j.nq = a.nq;
@@ -699,13 +738,18 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
         int occ = bfj_join_max_blocks_per_cu(p.dtype, g.kc, g.rows, g.lds);
         if (g.cap > 0) occ = std::min(occ, g.cap);
         const int persistent = std::max(1, occ) * ctx->n_cu;
-        L.grid_blocks = std::min(persistent, g.n_items);
+        L.grid_blocks = std::min(persistent, j.n_items);
         if (L.grid_blocks < 1) L.grid_blocks = 1;
         const int rc = bfj_join_launch(L, j, st);
         if (rc == 1) return fail(BFJ_ERR_INTERNAL, "join kernel dtype %d kc %d rows %d not built", p.dtype, g.kc, g.rows);
         if (rc) return fail(BFJ_ERR_DEVICE, "join launch failed: %s", hipGetErrorString(hipGetLastError()));
     }
-    if (timed) HC(hipEventRecord(ctx->ev[2], st));
+    return BFJ_OK;
+}
+
+// Band tests of the candidate stream, the middle stage and the FP64 refine.
+static void enqueue_downstream(const PassArgs &a, hipStream_t st)
+{
     bfj_launch_classify(a, st);
     if (a.mid) {
         bfj_launch_mid(a, st);
@@ -716,6 +760,21 @@ static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
     } else {
         bfj_launch_refine(a, st);
     }
+}
+
+// Enqueue the whole pass on the context stream (no host synchronisation, no
+// memset nodes: safe to capture in a graph).
+static int enqueue_pass(bfj_ctx *ctx, const PassPlan &p, bool timed)
+{
+    ctx->last.reset();
+    const PassArgs a = pass_args(ctx, p);
+    hipStream_t st = ctx->stream;
+    if (timed) HC(hipEventRecord(ctx->ev[0], st));
+    RC(enqueue_prep(ctx, p, a));
+    if (timed) HC(hipEventRecord(ctx->ev[1], st));
+    RC(enqueue_join(ctx, p, a, 0, p.items ? p.items->size() : 0));
+    if (timed) HC(hipEventRecord(ctx->ev[2], st));
+    enqueue_downstream(a, st);
     bfj_launch_finalize(a, st);
     HC(hipGetLastError());
     if (timed) HC(hipEventRecord(ctx->ev[3], st));
@@ -739,39 +798,43 @@ static int check_finite(const std::vector<SetScalars> &sc)
 
 // After a pass: if the candidate stream or the refine list overflowed (records
 // past the capacity were dropped), grow both to what the pass asked for and set
-// *again so that the caller repeats the pass.
+// *again so that the caller repeats the pass.  When the grown buffers would exceed
+// the stream limit (or a no-allocation call cannot grow them), set *slice instead:
+// the caller then runs the pass in slices (run_sliced).
 #define BFJ_MAX_CHUNKS (1ull << 26)   // 64 Mi chunks of 1024 candidate records (1 TiB): more is refused
 #define BFJ_MAX_BAND (1ull << 34)     // band-list slots (256 GiB per list): more is refused
-static int grow_on_overflow(bfj_ctx *ctx, const unsigned long long *ctr, bool *again)
+static int grow_on_overflow(bfj_ctx *ctx, const PassPlan &p, const unsigned long long *ctr, bool *again,
+                            bool *slice)
 {
     *again = false;
+    *slice = false;
     const unsigned long long band_count = ctr[BFJ_CTR_BAND], chunks = ctr[BFJ_CTR_CHUNKS];
-    if (ctx->noalloc && (chunks > ctx->chunk_capacity || band_count > ctx->band_capacity))
-        return fail(BFJ_ERR_CAPACITY,
-                    "the pass needs %llu candidate chunks and %llu band-list slots; the no-allocation scratch "
-                    "(%zu bytes) holds %u and %llu for this dataset",
-                    chunks, band_count, BFJ_SCRATCH_BYTES, ctx->chunk_capacity,
-                    (unsigned long long)ctx->band_capacity);
-    if (chunks > ctx->chunk_capacity) {
-        if (chunks > BFJ_MAX_CHUNKS)
-            return fail(BFJ_ERR_UNSUPPORTED, "the pass needs %llu candidate chunks (at most %llu)", chunks,
-                        BFJ_MAX_CHUNKS);
-        ctx->chunk_capacity = (uint32_t)std::min<unsigned long long>(chunks + chunks / 4 + 64, BFJ_MAX_CHUNKS);
+    const bool over_c = chunks > ctx->chunk_capacity, over_b = band_count > ctx->band_capacity;
+    if (!over_c && !over_b) return BFJ_OK;
+    const unsigned long long want_c =
+        over_c ? std::min<unsigned long long>(chunks + chunks / 4 + 64, BFJ_MAX_CHUNKS) : ctx->chunk_capacity;
+    const unsigned long long want_b =
+        over_b ? std::min<unsigned long long>(band_count + band_count / 2 + 1024, BFJ_MAX_BAND) : ctx->band_capacity;
+    const double bytes = (double)want_c * BFJ_CHUNK * sizeof(Candidate) +
+                         (double)want_b * sizeof(BandEntry) * (double)band_lists(ctx, p);
+    if (ctx->noalloc || bytes > (double)stream_limit(ctx) || chunks > BFJ_MAX_CHUNKS || band_count > BFJ_MAX_BAND) {
+        *slice = true;
+        return BFJ_OK;
+    }
+    if (over_c) {
+        ctx->chunk_capacity = (uint32_t)want_c;
         RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
         RC(ensure(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
         *again = true;
     }
-    if (band_count > ctx->band_capacity) {
-        if (band_count > BFJ_MAX_BAND)
-            return fail(BFJ_ERR_UNSUPPORTED, "the pass needs %llu band-list slots (at most %llu)", band_count,
-                        BFJ_MAX_BAND);
+    if (over_b) {
         // band_count counts reserved slots: the records plus the holes of classify's
         // last blocks.  The larger capacity may give larger blocks (pass_args); while
         // a block is above its 64-slot minimum the holes stay within a quarter of the
         // capacity, so 1.5x the slots holds the same pass again (records + capacity / 4
         // <= slots + capacity / 4 <= capacity).  At the minimum the holes are at most
         // 8192 x 64 slots, which the growth covers within a repeat or two more.
-        ctx->band_capacity = std::min<unsigned long long>(band_count + band_count / 2 + 1024, BFJ_MAX_BAND);
+        ctx->band_capacity = want_b;
         RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
         if (ctx->mid_band.p) RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
         *again = true;
@@ -821,34 +884,169 @@ static int read_counters(bfj_ctx *ctx, unsigned long long *ctr)
     return BFJ_OK;
 }
 
+// The totals and statistics of a pass that ran to completion; it becomes the
+// context's last pass.
+static int finish_pass(bfj_ctx *ctx, const PassPlan &p, const unsigned long long *ctr, int retries, bool timed,
+                       std::vector<std::vector<uint64_t>> *totals_out, bfj_stats *stats)
+{
+    std::vector<SetScalars> sc;
+    RC(collect_pass(ctx, p, ctr, retries, timed, sc, stats));
+    if (stats) stats->slices = 1;
+    ctx->last.reset(new PassPlan(p));
+    if (totals_out) {
+        totals_out->assign(p.sets.size(), {});
+        for (size_t si = 0; si < p.sets.size(); ++si)
+            (*totals_out)[si].assign(sc[si].total, sc[si].total + p.sets[si].n_eps);
+    }
+    return BFJ_OK;
+}
+
+// A pass whose candidate stream does not fit the stream limit (or the fixed buffers
+// of a no-allocation call): prep once, then the join's work items in slices, each
+// slice followed by its band tests, middle stage and FP64 refine, which add to the
+// same per-row counts; finalize once at the end.  Every pair is still decided by
+// exactly one record of one slice, so the counts are those of a single pass.
+//
+// The band lists are sized so that a slice whose join fits the candidate stream fits
+// them too, and no slice is repeated once its counts were added: a slice of c chunks
+// holds at most 1024c records, and k_classify leaves holes only in the last block of
+// each wavefront that had a chunk (min(c, 8192) blocks of band_block_for(B) slots).
+// With B = 4/3 x 1024K + 64 min(K, 8192) + 1024 slots for K chunks, the records and
+// holes of any c <= K chunks fit: at 64-slot blocks directly, and at larger blocks
+// the holes are within B/4 (band_block_for) and the records within 3B/4.  A slice
+// whose join needs more chunks than that is repeated with half its items before
+// anything else runs on it.  Host synchronisation after each slice's join and band
+// tests: eager only, never captured.  chunks_hint, the chunks the whole pass needs
+// (0: unknown), sizes the first slice.
+static int run_sliced(bfj_ctx *ctx, PassPlan &p, unsigned long long chunks_hint,
+                      std::vector<std::vector<uint64_t>> *totals_out, bfj_stats *stats, int retries)
+{
+    const uint64_t W = BFJ_CLASSIFY_WAVES;
+    if (!ctx->noalloc) {
+        // Candidate chunks K and band slots 4/3 x 1024K + 64 min(K, 8192) + 1024 per
+        // list, together within the stream limit.
+        const double lists = (double)band_lists(ctx, p);
+        const double per_chunk =
+            (double)BFJ_CHUNK * sizeof(Candidate) + lists * sizeof(BandEntry) * (BFJ_CHUNK * 4.0 / 3.0 + 64.0);
+        const double fixed = lists * sizeof(BandEntry) * 1025.0;
+        const double k = std::min(((double)stream_limit(ctx) - fixed) / per_chunk, (double)BFJ_MAX_CHUNKS);
+        if (k < 1.0)
+            return fail(BFJ_ERR_CAPACITY, "the stream limit (%llu bytes) holds no candidate chunk and its band lists",
+                        (unsigned long long)stream_limit(ctx));
+        ctx->chunk_capacity = (uint32_t)k;
+        ctx->band_capacity = ((uint64_t)ctx->chunk_capacity * BFJ_CHUNK * 4 + 2) / 3 +
+                             64 * std::min<uint64_t>(ctx->chunk_capacity, W) + 1024;
+        RC(stage_plan(ctx, p));
+    }
+    // Chunks a slice may fill: its records and holes within the band capacity B.
+    const uint64_t B = ctx->band_capacity, blk = band_block_for(B);
+    uint64_t kr = B / (BFJ_CHUNK + blk);
+    if (kr >= W) kr = (B - W * blk) / BFJ_CHUNK;
+    kr = std::min<uint64_t>(kr, ctx->chunk_capacity);
+    if (kr == 0)
+        return fail(BFJ_ERR_CAPACITY, "a band list of %llu slots holds no slice of the candidate stream",
+                    (unsigned long long)B);
+    ctx->last.reset();
+    const PassArgs a = pass_args(ctx, p);
+    hipStream_t st = ctx->stream;
+    const size_t n_items = p.items ? p.items->size() : 0;
+    HC(hipEventRecord(ctx->ev[0], st));
+    RC(enqueue_prep(ctx, p, a));
+    HC(hipEventRecord(ctx->ev[1], st));
+    static_assert(BFJ_CTR_BAND == 0 && BFJ_CTR_CHUNKS == 1 && BFJ_CTR_PAIRS == 2 && BFJ_CTR_MID == 3,
+                  "counter slots zeroed per slice");
+    unsigned long long *cdev = (unsigned long long *)ctx->counters.p;
+    unsigned long long tot[BFJ_CTR_ITEMS] = {};
+    double ms_join = 0.0, ms_down = 0.0;
+    int slices = 0;
+    // Items per slice: aimed at 3/4 of kr chunks.
+    size_t step = chunks_hint ? (size_t)std::max(1.0, 0.75 * (double)n_items * (double)kr / (double)chunks_hint)
+                              : n_items;
+    for (size_t lo = 0; lo < n_items;) {
+        size_t hi = std::min(n_items, lo + std::max<size_t>(step, 1));
+        unsigned long long ctr[BFJ_CTR_ITEMS] = {};
+        for (;;) {
+            // Every counter but the pair list's, which the slices share, starts at zero.
+            bfj_launch_zero(cdev, 16, st);
+            bfj_launch_zero(cdev + 3, 8, st);
+            bfj_launch_zero(cdev + 4, (BFJ_CTR_SLOTS - 4) * 8, st);
+            HC(hipEventRecord(ctx->ev[2], st));
+            RC(enqueue_join(ctx, p, a, lo, hi));
+            HC(hipEventRecord(ctx->ev[3], st));
+            RC(read_counters(ctx, ctr));
+            if (ctr[BFJ_CTR_CHUNKS] <= kr) break;
+            if (hi - lo == 1)
+                return fail(BFJ_ERR_CAPACITY,
+                            "one join work item streams %llu candidate chunks; a slice holds %llu (stream limit %llu "
+                            "bytes)",
+                            ctr[BFJ_CTR_CHUNKS], (unsigned long long)kr, (unsigned long long)stream_limit(ctx));
+            hi = lo + (hi - lo) / 2;
+            ++retries;
+        }
+        float t = 0.f;
+        HC(hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
+        ms_join += t;
+        enqueue_downstream(a, st);
+        HC(hipGetLastError());
+        HC(hipEventRecord(ctx->ev[4], st));
+        RC(read_counters(ctx, ctr));
+        HC(hipEventElapsedTime(&t, ctx->ev[3], ctx->ev[4]));
+        ms_down += t;
+        if (ctr[BFJ_CTR_BAND] > B || ctr[BFJ_CTR_MID] > B)
+            return fail(BFJ_ERR_INTERNAL, "a slice of %llu chunks overflowed its band list (%llu slots of %llu)",
+                        ctr[BFJ_CTR_CHUNKS], std::max(ctr[BFJ_CTR_BAND], ctr[BFJ_CTR_MID]), (unsigned long long)B);
+        for (int k : {BFJ_CTR_BAND, BFJ_CTR_CHUNKS, BFJ_CTR_MID, BFJ_CTR_CANDS, BFJ_CTR_BANDREC}) tot[k] += ctr[k];
+        ++slices;
+        const double grow = ctr[BFJ_CTR_CHUNKS] ? 0.75 * (double)kr / (double)ctr[BFJ_CTR_CHUNKS] : 4.0;
+        step = (size_t)std::max(1.0, (double)(hi - lo) * std::min(grow, 4.0));
+        lo = hi;
+    }
+    bfj_launch_finalize(a, st);
+    HC(hipGetLastError());
+    HC(hipEventRecord(ctx->ev[4], st));
+    HC(hipStreamSynchronize(st));
+    p.stream_chunks = tot[BFJ_CTR_CHUNKS];
+    RC(finish_pass(ctx, p, tot, retries, false, totals_out, stats));
+    if (stats) {
+        float t01 = 0.f, t04 = 0.f;
+        HC(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+        HC(hipEventElapsedTime(&t04, ctx->ev[0], ctx->ev[4]));
+        stats->ms_prep = t01;
+        stats->ms_join = ms_join;
+        stats->ms_refine = ms_down;
+        stats->ms_total = t04;
+        stats->slices = slices;
+    }
+    return BFJ_OK;
+}
+
 // Run a staged pass to completion, growing the candidate stream / refine list and
-// repeating the pass when one overflowed.  Totals per set land in totals_out[set][eps].
+// repeating the pass when one overflowed, or running it in slices when they would
+// outgrow the stream limit.  Totals per set land in totals_out[set][eps].
 static int run_pass(bfj_ctx *ctx, PassPlan &p, std::vector<std::vector<uint64_t>> *totals_out, bfj_stats *stats,
                     bool timed = true)
 {
     int retries = 0;
-    for (;;) {
+    unsigned long long hint = p.stream_chunks;
+    while (!p.sliced) {
         RC(enqueue_pass(ctx, p, timed));
         unsigned long long ctr[BFJ_CTR_ITEMS] = {};
         RC(read_counters(ctx, ctr));
-        bool again = false;
-        RC(grow_on_overflow(ctx, ctr, &again));
-        if (!again) {
-            std::vector<SetScalars> sc;
-            RC(collect_pass(ctx, p, ctr, retries, timed, sc, stats));
-            ctx->last.reset(new PassPlan(p));
-            if (totals_out) {
-                totals_out->assign(p.sets.size(), {});
-                for (size_t si = 0; si < p.sets.size(); ++si)
-                    (*totals_out)[si].assign(sc[si].total, sc[si].total + p.sets[si].n_eps);
-            }
-            return BFJ_OK;
+        bool again = false, slice = false;
+        RC(grow_on_overflow(ctx, p, ctr, &again, &slice));
+        if (slice) {
+            p.sliced = 1;
+            hint = ctr[BFJ_CTR_CHUNKS];
+            ++retries;
+            break;
         }
+        if (!again) return finish_pass(ctx, p, ctr, retries, timed, totals_out, stats);
         if (++retries > 4)
             return fail(BFJ_ERR_INTERNAL, "candidate stream / band list keep overflowing (%llu chunks, %llu slots)",
                         ctr[BFJ_CTR_CHUNKS], ctr[BFJ_CTR_BAND]);
         RC(stage_plan(ctx, p)); // the grown buffers may have moved
     }
+    return run_sliced(ctx, p, hint, totals_out, stats, retries);
 }
 
 // Uniform [0, 1) rows for the tile autotuner (splitmix64).
@@ -989,6 +1187,9 @@ int bfj_init(int device, bfj_ctx **out)
             if (e == hipSuccess) e = hipEventCreate(&ev);
     }
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    size_t mem = 0;
+    if (e == hipSuccess) e = hipDeviceTotalMem(&mem, device);
+    c->mem_total = mem;
     if (e != hipSuccess) {
         bfj_destroy(c);
         return fail(BFJ_ERR_DEVICE, "context setup failed: %s", hipGetErrorString(e));
@@ -1449,10 +1650,11 @@ int bfj_plan_create(bfj_ctx *ctx, const bfj_dataset *sets, int32_t n_sets, int32
     int rc = build_plan(ctx, hs, dtype, pl->pass, 0);
     if (rc == BFJ_OK) rc = stage_plan(ctx, pl->pass);
     if (rc == BFJ_OK) {
-        // One eager pass sizes the refine list, then the pass is captured.
+        // One eager pass sizes the refine list, then the pass is captured (unless it
+        // runs in slices, which are eager).
         rc = run_pass(ctx, pl->pass, nullptr, nullptr);
     }
-    if (rc == BFJ_OK) rc = capture(pl);
+    if (rc == BFJ_OK && !pl->pass.sliced) rc = capture(pl);
     if (rc != BFJ_OK) {
         bfj_plan_destroy(pl);
         return rc;
@@ -1468,6 +1670,18 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
     HC(hipSetDevice(ctx->device));
     // The context's buffers are shared with eager calls: restage if they moved.
     RC(stage_plan(ctx, pl->pass));
+    std::vector<std::vector<uint64_t>> tot;
+    auto give_totals = [&]() {
+        for (size_t i = 0; i < pl->sets.size(); ++i)
+            if (pl->sets[i].totals)
+                for (int e = 0; e < pl->sets[i].n_eps; ++e) pl->sets[i].totals[e] = tot[i][(size_t)e];
+    };
+    // A plan whose candidate stream outgrew the stream limit runs in slices, eagerly.
+    if (pl->pass.sliced) {
+        RC(run_pass(ctx, pl->pass, &tot, stats));
+        give_totals();
+        return BFJ_OK;
+    }
     if (pl->captured_gen != g_buf_gen.load() || pl->captured_chunks != ctx->chunk_capacity ||
         pl->captured_band != ctx->band_capacity)
         RC(capture(pl));
@@ -1476,15 +1690,17 @@ int bfj_plan_run(bfj_plan *pl, bfj_stats *stats)
         HC(hipGraphLaunch(pl->exec, ctx->stream));
         unsigned long long ctr[BFJ_CTR_ITEMS] = {};
         RC(read_counters(ctx, ctr));
-        bool again = false;
-        RC(grow_on_overflow(ctx, ctr, &again));
+        bool again = false, slice = false;
+        RC(grow_on_overflow(ctx, pl->pass, ctr, &again, &slice));
+        if (slice) {
+            pl->pass.sliced = 1;
+            RC(run_sliced(ctx, pl->pass, ctr[BFJ_CTR_CHUNKS], &tot, stats, attempt + 1));
+            give_totals();
+            return BFJ_OK;
+        }
         if (!again) {
-            std::vector<SetScalars> sc;
-            RC(collect_pass(ctx, pl->pass, ctr, attempt, true, sc, stats));
-            ctx->last.reset(new PassPlan(pl->pass));
-            for (size_t i = 0; i < pl->sets.size(); ++i)
-                if (pl->sets[i].totals)
-                    for (int e = 0; e < pl->sets[i].n_eps; ++e) pl->sets[i].totals[e] = sc[i].total[e];
+            RC(finish_pass(ctx, pl->pass, ctr, attempt, true, &tot, stats));
+            give_totals();
             return BFJ_OK;
         }
         RC(stage_plan(ctx, pl->pass));
@@ -1646,6 +1862,22 @@ int bfj_fail_allocation(int64_t countdown)
 {
     if (countdown < 0) return fail(BFJ_ERR_INVALID, "countdown < 0");
     g_fail_countdown.store(countdown);
+    return BFJ_OK;
+}
+
+int bfj_set_stream_limit(bfj_ctx *ctx, uint64_t bytes)
+{
+    if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
+    ctx->stream_limit = bytes;
+    // Stream buffers grown past the new limit are given back; the next pass sizes them
+    // within it.
+    if ((double)ctx->cand.cap + (double)ctx->band.cap + (double)ctx->mid_band.cap > (double)stream_limit(ctx)) {
+        HC(hipSetDevice(ctx->device));
+        HC(hipStreamSynchronize(ctx->stream));
+        ctx->last.reset();
+        for (DevBuf *b : {&ctx->cand, &ctx->chunk_fill, &ctx->band, &ctx->mid_band}) release(*b);
+        g_buf_gen.fetch_add(1);
+    }
     return BFJ_OK;
 }
 

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 14
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.bfj_abi_version() == bfj.ABI_VERSION == 3
+    assert lib.bfj_abi_version() == bfj.ABI_VERSION == 4
 
 
 def test_cli_exports_reference_entry_points():
