@@ -5,7 +5,8 @@ Both run the same kernels; the no-allocation call takes every working buffer fro
 file-scope device array instead of hipMalloc.  Per-point counts and totals must be
 identical (they are exact), on two calls in a row, the second on a smaller dataset
 (the array keeps the first call's contents), for every dtype; a dataset beyond the
-array's size is refused with BFJ_ERR_CAPACITY and leaves the context usable.
+array's size is refused with BFJ_ERR_CAPACITY and leaves the context usable, and a
+candidate stream beyond the array's share runs in slices within it.
 """
 import ctypes
 
@@ -81,3 +82,26 @@ def test_noalloc_refuses_what_does_not_fit(ctx):
     yd.free()
     res = ctx.self_join(y, eps, dtype="fp16")
     np.testing.assert_array_equal(res.counts, oc)
+
+
+@pytest.mark.parametrize("dtype", ["fp16", "fp64"])
+def test_noalloc_slices_a_stream_beyond_its_array(ctx, dtype):
+    """eps at the 40 % distance quantile: about 3.2e6 candidate records, more than the
+    array's share for the stream, so the call runs the join in slices within it.  The
+    counts equal the allocating call's and the oracle's."""
+    lib = ctx.lib
+    x = wl.uniform(4000, 64, 12)
+    eps = wl.eps_quantile(x, [0.05, 0.4], 300, 12)
+    xd = ctx.to_device(x)
+    rc0, c0, t0, s0 = _call(ctx, lib.bfj_self_join_device_noalloc, xd, 4000, 64, eps, dtype)
+    assert rc0 == 0, lib.bfj_last_error()
+    assert s0["slices"] >= 2, s0
+    rc1, c1, t1, s1 = _call(ctx, lib.bfj_self_join_device, xd, 4000, 64, eps, dtype)
+    assert rc1 == 0, lib.bfj_last_error()
+    np.testing.assert_array_equal(c0, c1)
+    np.testing.assert_array_equal(t0, t1)
+    assert s0["candidates"] == s1["candidates"] and s0["band_pairs"] == s1["band_pairs"]
+    oc, ot = oracle.join(x, eps)
+    np.testing.assert_array_equal(c0, oc)
+    np.testing.assert_array_equal(t0, ot)
+    xd.free()
