@@ -31,6 +31,7 @@ typedef float bfj_f4 __attribute__((ext_vector_type(4)));
 typedef double bfj_d2 __attribute__((ext_vector_type(2)));
 typedef double bfj_d4 __attribute__((ext_vector_type(4)));
 typedef int bfj_i8 __attribute__((ext_vector_type(8)));
+typedef int bfj_i4 __attribute__((ext_vector_type(4)));
 
 enum { BFJ_FP64 = 0, BFJ_FP16 = 1, BFJ_BF16 = 2, BFJ_FP8 = 3 };
 

@@ -169,7 +169,8 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / BFJ_WAVE));
     const int64_t nblk = s.npad / 16;
     const int wrb0 = rb0 + wave * RB; // first row block of this wavefront
-    const frag_t *P = reinterpret_cast<const frag_t *>(a.pack + s.pack_off);
+    const unsigned char *P = a.pack + s.pack_off; // fragment f starts at byte f * 64 * FB
+    constexpr int FB = (int)sizeof(frag_t);
     const val_t *nq = reinterpret_cast<const val_t *>(a.nq) + s.row_off;
     const val_t *gq = reinterpret_cast<const val_t *>(a.gq) + s.row_off;
     const uint8_t *sc = (Tr::dtype == BFJ_FP8) ? a.scales + s.row_off : nullptr;
@@ -185,7 +186,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         const bool ok = blk < nblk;
         any_rows |= ok;
 #pragma unroll
-        for (int c = 0; c < KC; ++c) A[rb][c] = ok ? P[(blk * KC + c) * BFJ_WAVE + lane] : frag_t{};
+        for (int c = 0; c < KC; ++c) A[rb][c] = ok ? Tr::load(P + (blk * KC + c) * BFJ_WAVE * FB, lane) : frag_t{};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int64_t row = blk * 16 + Tr::out_row(lane, r);
@@ -200,16 +201,22 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
     // unit 0 into the padding); wavefront 0 also loads the slot's norms and (FP8)
     // the last wavefront its scales.
     typedef __attribute__((address_space(3))) void lds_void;
-    const unsigned char *srcb = reinterpret_cast<const unsigned char *>(P);
+    const unsigned char *srcb = P;
     const unsigned char *nqb = reinterpret_cast<const unsigned char *>(nq);
     static_assert(L::NQB <= 4 * BFJ_WAVE, "one 64-lane load carries a slot's norms");
     const bool nq_wave = wave == 0;
     const bool sc_wave = Tr::dtype == BFJ_FP8 && wave == BFJ_JWAVES - 1;
+    // Ring unit U of a slot (16 bytes, in LDS order) is half h of lane operand l of
+    // fragment f (mfma_ops.h: the ring keeps fragments in 16-byte halves), which the
+    // packed layout holds at unit f * 64H + l H + h, H = FB / 16 halves per operand.
     uint32_t voff[L::UPW];                   // this lane's byte offset in a slot, per load
 #pragma unroll
     for (int u = 0; u < L::UPW; ++u) {
+        constexpr int H = FB / 16;
         const int unit = u * BFJ_JBLOCK + wave * BFJ_WAVE + lane;
-        voff[u] = (uint32_t)(unit < L::UNITS ? unit : 0) * 16u;
+        const int f = unit / (BFJ_WAVE * H), w = unit % (BFJ_WAVE * H);
+        const int src = f * BFJ_WAVE * H + (w % BFJ_WAVE) * H + w / BFJ_WAVE;
+        voff[u] = (uint32_t)(unit < L::UNITS ? src : 0) * 16u;
     }
     // Slot t of the item holds blocks cb0 + t*CPT ... + CPT-1; t and slot are uniform,
     // so the addresses are a scalar base plus a lane offset.  Blocks past the
@@ -343,7 +350,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         int SBn = 127;
         auto fetch = [&](int cb) {
 #pragma unroll
-            for (int c = 0; c < KC; ++c) Bn[c] = P[((int64_t)cb * KC + c) * BFJ_WAVE + lane];
+            for (int c = 0; c < KC; ++c) Bn[c] = Tr::load(P + ((int64_t)cb * KC + c) * BFJ_WAVE * FB, lane);
             njn = nq[cb * 16 + (lane & 15)];
             if (Tr::dtype == BFJ_FP8) SBn = (int)sc[cb * 16 + (lane & 15)];
         };
@@ -385,7 +392,7 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
         issue(t + L::NB - 1, rcur == 0 ? L::NB - 1 : rcur - 1); // into the buffer of slot t-1
         if (!any_rows) return;
         const unsigned char *b = lds + rcur * L::BUF;
-        auto frag = [&](int h, int c) { return reinterpret_cast<const frag_t *>(b)[(h * KC + c) * BFJ_WAVE + lane]; };
+        auto frag = [&](int h, int c) { return Tr::load_ring(b + (h * KC + c) * BFJ_WAVE * FB, lane); };
         auto norm = [&](int h) { return reinterpret_cast<const val_t *>(b + L::FRAG)[h * 16 + (lane & 15)]; };
         auto scale = [&](int h) {
             return (Tr::dtype == BFJ_FP8) ? (int)b[L::FRAG + L::NQB + h * 16 + (lane & 15)] : 127;
