@@ -82,18 +82,15 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_order_label(PassArgs a)
     ord_f4 acc[BFJ_PIVOTS / 16];
 #pragma unroll
     for (int t = 0; t < BFJ_PIVOTS / 16; ++t) acc[t] = ord_f4{0.f, 0.f, 0.f, 0.f};
-    for (int c = 0; c < nch; ++c) {
-        __syncthreads(); // previous chunk consumed
-        // Pivot columns [32c, 32c + 32) of every pivot: 128 x 32 halves.
-        for (int t = threadIdx.x; t < BFJ_PIVOTS * 4; t += BFJ_BLOCK) {
-            const int p = t >> 2, q = t & 3;
-            *(ord_h8 *)(pch + p * 32 + q * 8) = *(const ord_h8 *)(pv + p * BFJ_MAXDIM + c * 32 + q * 8);
-        }
-        __syncthreads();
-        // A fragment: row r, columns 32c + 8 (lane / 16) + j; 16-byte loads when the
-        // row's 8 values are whole and aligned.
-        ord_h8 av;
+    // A fragments of every K-chunk first (row r, columns 32c + 8 (lane / 16) + j), so that
+    // all of the row's loads are in flight together rather than one chunk per pair of
+    // barriers; 16-byte loads when the row's 8 values are whole and aligned.
+    ord_h8 avs[BFJ_MAXDIM / 32];
+#pragma unroll
+    for (int c = 0; c < BFJ_MAXDIM / 32; ++c) {
+        if (c >= nch) break; // uniform
         const int k0 = c * 32 + 8 * (lane >> 4);
+        ord_h8 av;
         if (real && s.al16 && k0 + 8 <= s.d) {
             const double2 *src = reinterpret_cast<const double2 *>(s.x + r * s.ld + k0);
 #pragma unroll
@@ -110,10 +107,22 @@ __global__ __launch_bounds__(BFJ_BLOCK) void k_order_label(PassArgs a)
                 av[j] = (_Float16)v;
             }
         }
+        avs[c] = av;
+    }
+#pragma unroll
+    for (int c = 0; c < BFJ_MAXDIM / 32; ++c) {
+        if (c >= nch) break; // uniform
+        __syncthreads(); // previous chunk consumed
+        // Pivot columns [32c, 32c + 32) of every pivot: 128 x 32 halves.
+        for (int t = threadIdx.x; t < BFJ_PIVOTS * 4; t += BFJ_BLOCK) {
+            const int p = t >> 2, q = t & 3;
+            *(ord_h8 *)(pch + p * 32 + q * 8) = *(const ord_h8 *)(pv + p * BFJ_MAXDIM + c * 32 + q * 8);
+        }
+        __syncthreads();
 #pragma unroll
         for (int t = 0; t < BFJ_PIVOTS / 16; ++t) {
             const ord_h8 bv = *(const ord_h8 *)(pch + (t * 16 + (lane & 15)) * 32 + 8 * (lane >> 4));
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(avs[c], bv, acc[t], 0, 0, 0);
         }
     }
     // acc[t][i] of lane l: row 4 (l / 16) + i of the wavefront's 16, pivot 16 t + l % 16.
