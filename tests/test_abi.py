@@ -68,6 +68,8 @@ def test_validation_without_device():
     assert rc == 1 and b"ctx" in lib.bfj_last_error()
     rc = lib.bfj_self_join_batch(None, None, 0, 1, 0, None)
     assert rc == 1
+    assert lib.bfj_set_stream_limit(None, 1 << 30) == 1 and b"ctx" in lib.bfj_last_error()
+    assert lib.bfj_set_capacity(None, 1, 1) == 1
     n = ctypes.c_int(-1)
     assert lib.bfj_device_count(ctypes.byref(n)) == 0 and n.value >= 0
     if n.value == 0:
