@@ -240,7 +240,9 @@ int bfj_set_capacity(bfj_ctx *ctx, uint64_t candidate_chunks, uint64_t band_reco
  * within its fixed buffers the same way.  Sliced passes synchronise with the host
  * after every slice and are not captured: a plan that needed slices runs them
  * eagerly from then on.  A limit too small for one work item's candidates fails with
- * BFJ_ERR_CAPACITY. */
+ * BFJ_ERR_CAPACITY.  A growth of the stream that the device cannot allocate (other
+ * allocations hold its memory) also runs the pass in slices, under half the bytes that
+ * failed, until this function is called again. */
 int bfj_set_stream_limit(bfj_ctx *ctx, uint64_t bytes);
 
 /* Fault injection for tests of the error paths (no counterpart in the reference):

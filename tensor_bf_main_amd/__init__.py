@@ -29,6 +29,7 @@ MAX_EPS = 8
 MAX_DIM = 256
 
 BFJ_OK = 0
+BFJ_ERR_NOMEM = 3
 BFJ_ERR_CAPACITY = 6
 STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "DEVICE", 3: "NOMEM", 4: "UNSUPPORTED", 5: "INTERNAL", 6: "CAPACITY"}
 

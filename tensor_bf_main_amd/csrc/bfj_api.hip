@@ -228,6 +228,7 @@ struct bfj_ctx {
     uint32_t chunk_capacity = 0; // candidate chunks of BFJ_CHUNK records
     uint64_t stream_limit = 0;   // bytes the candidate stream and band lists may take (0: half of mem_total)
     uint64_t mem_total = 0;      // the device's memory
+    uint64_t mem_ceiling = 0;    // stream bytes known not to allocate (a failed growth; 0: none)
     int tile_rows[4][33] = {};  // [dtype][kc] forced/tuned R (0 = heuristic)
     int tile_cap[4][33] = {};   // [dtype][kc] tuned resident blocks per CU (0 = as many as fit)
     int tile_panel[4][33] = {}; // [dtype][kc] tuned panel bytes (0 = BFJ_PANEL_BYTES)
@@ -528,7 +529,8 @@ static int build_plan(bfj_ctx *ctx, const std::vector<HostSet> &hs, int dtype, P
 // Bytes the candidate stream and the band lists of a pass may take.
 static uint64_t stream_limit(const bfj_ctx *ctx)
 {
-    return ctx->stream_limit ? ctx->stream_limit : ctx->mem_total / 2;
+    const uint64_t lim = ctx->stream_limit ? ctx->stream_limit : ctx->mem_total / 2;
+    return ctx->mem_ceiling ? std::min(lim, ctx->mem_ceiling) : lim;
 }
 
 // Band lists of a pass: the band list, and the middle stage's list when it runs.
@@ -821,13 +823,23 @@ static int grow_on_overflow(bfj_ctx *ctx, const PassPlan &p, const unsigned long
         *slice = true;
         return BFJ_OK;
     }
+    // A growth that the device cannot allocate (other allocations hold its memory) runs
+    // the pass in slices instead, under a ceiling of half the bytes that failed.
+    auto grow = [&](DevBuf &b, size_t n) {
+        const int rc = ensure(b, n);
+        if (rc == BFJ_ERR_NOMEM) {
+            ctx->mem_ceiling = (uint64_t)(bytes / 2);
+            *slice = true;
+        }
+        return rc == BFJ_ERR_NOMEM ? BFJ_OK : rc;
+    };
     if (over_c) {
         ctx->chunk_capacity = (uint32_t)want_c;
-        RC(ensure(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
-        RC(ensure(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
+        RC(grow(ctx->cand, (size_t)ctx->chunk_capacity * BFJ_CHUNK * sizeof(Candidate)));
+        if (!*slice) RC(grow(ctx->chunk_fill, (size_t)ctx->chunk_capacity * 4));
         *again = true;
     }
-    if (over_b) {
+    if (over_b && !*slice) {
         // band_count counts reserved slots: the records plus the holes of classify's
         // last blocks.  The larger capacity may give larger blocks (pass_args); while
         // a block is above its 64-slot minimum the holes stay within a quarter of the
@@ -835,10 +847,11 @@ static int grow_on_overflow(bfj_ctx *ctx, const PassPlan &p, const unsigned long
         // <= slots + capacity / 4 <= capacity).  At the minimum the holes are at most
         // 8192 x 64 slots, which the growth covers within a repeat or two more.
         ctx->band_capacity = want_b;
-        RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
-        if (ctx->mid_band.p) RC(ensure(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+        RC(grow(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
+        if (ctx->mid_band.p && !*slice) RC(grow(ctx->mid_band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
         *again = true;
     }
+    if (*slice) *again = false;
     return BFJ_OK;
 }
 
@@ -922,7 +935,9 @@ static int run_sliced(bfj_ctx *ctx, PassPlan &p, unsigned long long chunks_hint,
                       std::vector<std::vector<uint64_t>> *totals_out, bfj_stats *stats, int retries)
 {
     const uint64_t W = BFJ_CLASSIFY_WAVES;
-    if (!ctx->noalloc) {
+    // Buffers the device cannot allocate halve the ceiling and are tried again (at most
+    // 8 times, then the out-of-memory error stands).
+    for (int tries = 0; !ctx->noalloc; ++tries) {
         // Candidate chunks K and band slots 4/3 x 1024K + 64 min(K, 8192) + 1024 per
         // list, together within the stream limit.
         const double lists = (double)band_lists(ctx, p);
@@ -936,7 +951,10 @@ static int run_sliced(bfj_ctx *ctx, PassPlan &p, unsigned long long chunks_hint,
         ctx->chunk_capacity = (uint32_t)k;
         ctx->band_capacity = ((uint64_t)ctx->chunk_capacity * BFJ_CHUNK * 4 + 2) / 3 +
                              64 * std::min<uint64_t>(ctx->chunk_capacity, W) + 1024;
-        RC(stage_plan(ctx, p));
+        const int rc = stage_plan(ctx, p);
+        if (rc == BFJ_OK) break;
+        if (rc != BFJ_ERR_NOMEM || tries == 7) return rc;
+        ctx->mem_ceiling = stream_limit(ctx) / 2;
     }
     // Chunks a slice may fill: its records and holes within the band capacity B.
     const uint64_t B = ctx->band_capacity, blk = band_block_for(B);
@@ -1869,6 +1887,7 @@ int bfj_set_stream_limit(bfj_ctx *ctx, uint64_t bytes)
 {
     if (!ctx) return fail(BFJ_ERR_INVALID, "ctx is NULL");
     ctx->stream_limit = bytes;
+    ctx->mem_ceiling = 0;
     // Stream buffers grown past the new limit are given back; the next pass sizes them
     // within it.
     if ((double)ctx->cand.cap + (double)ctx->band.cap + (double)ctx->mid_band.cap > (double)stream_limit(ctx)) {

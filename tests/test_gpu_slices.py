@@ -154,3 +154,40 @@ def test_limit_too_small_is_refused(own_ctx):
     np.testing.assert_array_equal(res.counts, oc)
     np.testing.assert_array_equal(res.totals, ot)
     assert res.stats["slices"] == 1
+
+
+def test_growth_that_cannot_allocate_runs_in_slices(own_ctx):
+    """A pass whose candidate stream must grow, with the k-th device allocation from
+    then on failing as out-of-memory: a failed growth of the stream falls back to a
+    sliced pass under half the bytes that failed (exact counts), and any other failed
+    allocation reports BFJ_ERR_NOMEM and leaves the context usable."""
+    lib = own_ctx.lib
+    x, eps = wl.config(3, n=9000)
+    oc, ot = oracle.join(x, eps)
+    one = own_ctx.self_join(x, eps, dtype="fp16")
+    sliced = 0
+    try:
+        for k in range(1, 9):
+            # Stream buffers given back (a limit below them), the default limit again,
+            # and capacities that the pass must outgrow.
+            own_ctx.set_stream_limit(1)
+            own_ctx.set_stream_limit(0)
+            own_ctx.set_capacity(candidate_chunks=1, band_records=8)
+            assert lib.bfj_fail_allocation(k) == 0
+            try:
+                res = own_ctx.self_join(x, eps, dtype="fp16")
+            except bfj.JoinError as err:
+                assert err.code == bfj.BFJ_ERR_NOMEM and "injected" in str(err), str(err)
+                continue
+            finally:
+                lib.bfj_fail_allocation(0)
+            np.testing.assert_array_equal(res.counts, oc, err_msg=f"k = {k}")
+            np.testing.assert_array_equal(res.totals, ot)
+            _same_stats(one.stats, res.stats)
+            sliced += res.stats["slices"] >= 2
+    finally:
+        lib.bfj_fail_allocation(0)
+    assert sliced >= 1
+    own_ctx.set_stream_limit(0)
+    res = own_ctx.self_join(x, eps, dtype="fp16")
+    np.testing.assert_array_equal(res.counts, oc)
