@@ -219,7 +219,7 @@ struct bfj_ctx {
     hipEvent_t ev[5] = {};
     DevBuf pack, nq, gq, he, row_set, qerr, dnq, nrm, scales, cnt, band, counters, colsum, mu, sets, scal, items, tabs, xcopy,
         outbuf, cand, chunk_fill, pairs, ref_x, ref_aux, perm, label, ohist, oseg, pivots, pivot_norm, mid_rows,
-        mid_q, mid_band;
+        mid_q, mid_band, blk_tot;
     int reorder = 1;   // pack whole-dataset passes in a locality order (order.hip)
     int mid_stage = 1; // low-precision passes: middle stage before the FP64 refine (refine.hip)
     uint64_t band_capacity = 0;  // records of the band list and of the middle stage's list
@@ -599,6 +599,7 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
     const size_t nb = p.blk_set.size(), ns = p.sets.size();
     const size_t tab_bytes = nb * 4 + nb * 8 + ns * 4 * 2 + 64;
     RC(ensure(ctx->tabs, tab_bytes));
+    RC(ensure(ctx->blk_tot, std::max<size_t>(nb, 1) * BFJ_MAXEPS * 8));
     int max_nblk = 1;
     for (int32_t v : p.set_nblk) max_nblk = std::max(max_nblk, (int)v);
     const size_t max_seg = (size_t)(max_nblk + BFJ_ORD_SEG - 1) / BFJ_ORD_SEG;
@@ -643,6 +644,7 @@ static PassArgs pass_args(bfj_ctx *ctx, const PassPlan &p)
     unsigned char *t = (unsigned char *)ctx->tabs.p;
     a.sets = (const SetDesc *)ctx->sets.p;
     a.scal = (SetScalars *)ctx->scal.p;
+    a.blk_tot = (unsigned long long *)ctx->blk_tot.p;
     a.n_sets = (int32_t)ns;
     a.dtype = p.dtype;
     a.blk_row0 = (const int64_t *)t;
@@ -1087,7 +1089,7 @@ static std::vector<DevBuf *> all_bufs(bfj_ctx *c)
             &c->band, &c->counters, &c->colsum, &c->mu, &c->sets, &c->scal, &c->items, &c->tabs,
             &c->xcopy, &c->outbuf, &c->cand, &c->chunk_fill, &c->pairs, &c->ref_x, &c->ref_aux,
             &c->perm, &c->label, &c->ohist, &c->oseg, &c->pivots, &c->pivot_norm, &c->mid_rows,
-            &c->mid_q, &c->mid_band, &c->errbuf};
+            &c->mid_q, &c->mid_band, &c->errbuf, &c->blk_tot};
 }
 
 // This device's scratch address, looked up once per device with the device current.

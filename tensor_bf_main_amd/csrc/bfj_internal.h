@@ -158,6 +158,7 @@ __host__ __device__ inline int bfj_acc_bytes(int dtype) { return dtype == BFJ_FP
 struct PassArgs {
     const SetDesc *sets;       // device
     SetScalars *scal;          // device, one per set
+    unsigned long long *blk_tot; // [row block][BFJ_MAXEPS] totals of each block (k_finalize)
     int32_t n_sets;
     int32_t dtype;
     const int32_t *blk_set;    // block -> set table for row-parallel kernels

@@ -56,10 +56,15 @@ void bfj_launch_zero(void *p, size_t bytes, hipStream_t s)
                        (unsigned char *)p + n16 * 16, ntail);
 }
 
+// Blocks of one dataset all raise the same word, and same-address atomics are performed
+// one after another in memory; a block whose value the word already holds skips its
+// atomic.  The word only grows, so a stale read can only lead to an unneeded atomic.
 __device__ inline void atomic_max_nonneg(unsigned long long *addr, double v)
 {
     if (!(v >= 0.0)) return;
-    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    if (__hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= b) return;
+    atomicMax(addr, b);
 }
 
 __device__ inline double wave_sum(double v)

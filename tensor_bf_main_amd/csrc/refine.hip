@@ -608,16 +608,34 @@ __global__ void k_finalize(PassArgs a)
         red[e][threadIdx.x] = c;
     }
     __syncthreads();
-    if (threadIdx.x < (unsigned)s.n_eps) {
+    // The block's totals go to its own slot (k_totals adds them up per dataset): one
+    // same-address atomic per block would be performed one after another in memory.
+    if (threadIdx.x < BFJ_MAXEPS) {
         unsigned long long t = 0;
-        for (int r = 0; r < BFJ_PREP_ROWS; ++r) t += red[threadIdx.x][r];
-        if (t) atomicAdd(&a.scal[si].total[s.eperm[threadIdx.x]], t);
+        if (threadIdx.x < (unsigned)s.n_eps)
+            for (int r = 0; r < BFJ_PREP_ROWS; ++r) t += red[threadIdx.x][r];
+        a.blk_tot[(int64_t)b * BFJ_MAXEPS + (threadIdx.x < (unsigned)s.n_eps ? s.eperm[threadIdx.x] : threadIdx.x)] = t;
+    }
+}
+
+// One wavefront per dataset: its totals, the sums of its row blocks' slots.
+__global__ __launch_bounds__(BFJ_WAVE) void k_totals(PassArgs a)
+{
+    const int si = blockIdx.x, lane = threadIdx.x;
+    const int n_eps = a.sets[si].n_eps;
+    const int64_t b0 = a.set_blk0[si], nb = a.set_nblk[si];
+    for (int e = 0; e < n_eps; ++e) {
+        unsigned long long t = 0;
+        for (int64_t k = lane; k < nb; k += BFJ_WAVE) t += a.blk_tot[(b0 + k) * BFJ_MAXEPS + e];
+        for (int o = BFJ_WAVE / 2; o > 0; o >>= 1) t += __shfl_xor(t, o);
+        if (lane == 0) a.scal[si].total[e] = t;
     }
 }
 
 void bfj_launch_finalize(const PassArgs &a, hipStream_t st)
 {
     hipLaunchKernelGGL(k_finalize, dim3(a.n_row_blocks), dim3(BFJ_PREP_ROWS), 0, st, a);
+    if (a.n_sets > 0) hipLaunchKernelGGL(k_totals, dim3(a.n_sets), dim3(BFJ_WAVE), 0, st, a);
 }
 
 // ---- middle stage ---------------------------------------------------------------------
