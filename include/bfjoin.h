@@ -38,9 +38,10 @@ extern "C" {
 /* 2: bfj_stats gained fp64_pairs.  3: bfj_stats lost max_abs_d2_err (see bfj_pass_errors)
    and gained candidates; bfj_set_capacity, bfj_fail_allocation, bfj_pass_errors,
    bfj_plan_tiles and bfj_self_join_device_noalloc.  4: bfj_stats gained slices;
-   bfj_set_stream_limit.  (Another struct layout: callers built against an earlier
-   version must be rebuilt.) */
-#define BFJ_ABI_VERSION 4
+   bfj_set_stream_limit.  5: candidate chunks of BFJ_CHUNK_RECORDS = 4096 records
+   (bfj_set_capacity counted chunks of 1024 before).  (Another struct layout or unit:
+   callers built against an earlier version must be rebuilt.) */
+#define BFJ_ABI_VERSION 5
 
 typedef struct bfj_ctx bfj_ctx;
 typedef struct bfj_plan bfj_plan;
@@ -65,6 +66,7 @@ typedef enum bfj_dtype {
 
 #define BFJ_MAX_EPS 8   /* epsilon values per dataset in one pass */
 #define BFJ_MAX_DIM 256 /* largest d the MFMA kernels are built for */
+#define BFJ_CHUNK_RECORDS 4096 /* candidate records (16 bytes each) per chunk of the candidate stream */
 
 typedef struct bfj_stats {
     uint64_t pairs;          /* ordered pairs evaluated: (row_end - row_begin) * n, summed over datasets */
@@ -224,7 +226,7 @@ int bfj_set_tile_rows(bfj_ctx *ctx, int32_t dtype, int32_t tile_rows);
  * eps fall into few tiles.  Results do not depend on it; enable = 0 keeps input order. */
 int bfj_set_reorder(bfj_ctx *ctx, int32_t enable);
 
-/* Capacities of the candidate stream (chunks of 1024 records) and of the band lists
+/* Capacities of the candidate stream (chunks of BFJ_CHUNK_RECORDS records) and of the band lists
  * (records) for the next pass; 0 leaves one as it is.  No counterpart in the
  * reference, which has neither.  A pass that needs more reports it, and the library
  * grows the buffer and repeats the pass (stats band_retries): results never depend

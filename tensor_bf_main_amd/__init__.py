@@ -95,7 +95,8 @@ class Dataset(ctypes.Structure):
 
 
 _lib = None
-ABI_VERSION = 4  # BFJ_ABI_VERSION of include/bfjoin.h that Stats and the prototypes mirror
+ABI_VERSION = 5  # BFJ_ABI_VERSION of include/bfjoin.h that Stats and the prototypes mirror
+CHUNK_RECORDS = 4096  # BFJ_CHUNK_RECORDS: candidate records (16 bytes each) per stream chunk
 
 
 def load_library(path: str = LIB_PATH):
@@ -425,7 +426,7 @@ class Context:
         _check(self.lib.bfj_set_tile_rows(self._h, _dtype_id(dtype), rows))
 
     def set_capacity(self, candidate_chunks: int = 0, band_records: int = 0):
-        """Capacities of the candidate stream (chunks of 1024 records) and the band lists for
+        """Capacities of the candidate stream (chunks of CHUNK_RECORDS records) and the band lists for
         the next pass (0 leaves one as it is).  A pass that needs more grows them and is
         repeated, so results never depend on these."""
         _check(self.lib.bfj_set_capacity(self._h, int(candidate_chunks), int(band_records)))

@@ -14,6 +14,7 @@
 //     uint64 (:318-321)                       status codes instead of exit(1)
 #include "bfj_internal.h"
 #include "../../include/bfjoin.h"
+static_assert(BFJ_CHUNK == BFJ_CHUNK_RECORDS, "candidate chunk size of the kernels and of the public header");
 
 #include <algorithm>
 #include <atomic>
@@ -575,10 +576,11 @@ static int stage_plan(bfj_ctx *ctx, PassPlan &p)
         ctx->band_capacity = std::min<uint64_t>(std::max<uint64_t>((uint64_t)p.rows_total * 64, 1u << 20), 1u << 28);
     if (ctx->chunk_capacity == 0) {
         // Start at 96 candidates per row (1.5 GiB at a million rows; about 100 neighbours
-        // per point fit) and at least 4 chunks per join wavefront of the GPU; a pass
-        // that needs more is repeated with the size it reported.
-        const uint64_t want = std::max<uint64_t>((uint64_t)p.rows_total * 96 / BFJ_CHUNK, 16384);
-        ctx->chunk_capacity = (uint32_t)std::min<uint64_t>(want, 1u << 24);
+        // per point fit) and at least 16 Mi records (256 MiB: two chunks for each of
+        // 2048 join wavefronts); a pass that needs more is repeated with the size it
+        // reported.
+        const uint64_t want = std::max<uint64_t>((uint64_t)p.rows_total * 96, 16ull << 20) / BFJ_CHUNK;
+        ctx->chunk_capacity = (uint32_t)std::min<uint64_t>(want, (1ull << 34) / BFJ_CHUNK);
     }
     clamp_to_limit(ctx, p);
     RC(ensure(ctx->band, (size_t)ctx->band_capacity * sizeof(BandEntry)));
@@ -805,7 +807,7 @@ static int check_finite(const std::vector<SetScalars> &sc)
 // *again so that the caller repeats the pass.  When the grown buffers would exceed
 // the stream limit (or a no-allocation call cannot grow them), set *slice instead:
 // the caller then runs the pass in slices (run_sliced).
-#define BFJ_MAX_CHUNKS (1ull << 26)   // 64 Mi chunks of 1024 candidate records (1 TiB): more is refused
+#define BFJ_MAX_CHUNKS ((1ull << 36) / BFJ_CHUNK) // 64 Gi candidate records (1 TiB): more is refused
 #define BFJ_MAX_BAND (1ull << 34)     // band-list slots (256 GiB per list): more is refused
 static int grow_on_overflow(bfj_ctx *ctx, const PassPlan &p, const unsigned long long *ctr, bool *again,
                             bool *slice)
@@ -924,9 +926,9 @@ static int finish_pass(bfj_ctx *ctx, const PassPlan &p, const unsigned long long
 //
 // The band lists are sized so that a slice whose join fits the candidate stream fits
 // them too, and no slice is repeated once its counts were added: a slice of c chunks
-// holds at most 1024c records, and k_classify leaves holes only in the last block of
-// each wavefront that had a chunk (min(c, 8192) blocks of band_block_for(B) slots).
-// With B = 4/3 x 1024K + 64 min(K, 8192) + 1024 slots for K chunks, the records and
+// holds at most cC records (C = BFJ_CHUNK), and k_classify leaves holes only in the last
+// block of each wavefront that had a chunk (min(c, 8192) blocks of band_block_for(B)
+// slots).  With B = 4/3 x CK + 64 min(K, 8192) + 1024 slots for K chunks, the records and
 // holes of any c <= K chunks fit: at 64-slot blocks directly, and at larger blocks
 // the holes are within B/4 (band_block_for) and the records within 3B/4.  A slice
 // whose join needs more chunks than that is repeated with half its items before
@@ -940,7 +942,7 @@ static int run_sliced(bfj_ctx *ctx, PassPlan &p, unsigned long long chunks_hint,
     // Buffers the device cannot allocate halve the ceiling and are tried again (at most
     // 8 times, then the out-of-memory error stands).
     for (int tries = 0; !ctx->noalloc; ++tries) {
-        // Candidate chunks K and band slots 4/3 x 1024K + 64 min(K, 8192) + 1024 per
+        // Candidate chunks K and band slots 4/3 x CK + 64 min(K, 8192) + 1024 per
         // list, together within the stream limit.
         const double lists = (double)band_lists(ctx, p);
         const double per_chunk =
@@ -1323,7 +1325,9 @@ int bfj_self_join_device_noalloc(bfj_ctx *ctx, const double *d_data, int64_t n, 
         for (int e = 0; e < n_eps; ++e) h.eps[e] = eps[e];
         PassPlan p;
         RC(build_plan(ctx, {h}, dtype, p, 0));
-        ctx->chunk_capacity = 64;
+        // The smallest stream a no-allocation pass runs with: 64 Ki records.
+        const size_t min_chunks = ((size_t)1 << 16) / BFJ_CHUNK;
+        ctx->chunk_capacity = (uint32_t)min_chunks;
         ctx->band_capacity = 1024;
         const int rc = stage_plan(ctx, p);
         if (rc == BFJ_ERR_CAPACITY)
@@ -1336,7 +1340,7 @@ int bfj_self_join_device_noalloc(bfj_ctx *ctx, const double *d_data, int64_t n, 
         const size_t cand_share = p.dtype == BFJ_FP8 ? rest / 3 : rest / 10 * 6;
         const size_t chunks = cand_share / ((size_t)BFJ_CHUNK * sizeof(Candidate) + 4);
         const size_t band = (rest - cand_share) / (lists * sizeof(BandEntry));
-        if (chunks < 64 || band < 1024)
+        if (chunks < min_chunks || band < 1024)
             return fail(BFJ_ERR_CAPACITY, "n = %lld, d = %d leaves too little of the %zu-byte no-allocation scratch "
                         "for the candidate stream", (long long)n, d, BFJ_SCRATCH_BYTES);
         ctx->chunk_capacity = (uint32_t)std::min<size_t>(chunks, BFJ_MAX_CHUNKS);

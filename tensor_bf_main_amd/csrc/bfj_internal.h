@@ -103,7 +103,10 @@ struct Candidate {
 };
 #define BFJ_COL_SYM 0x80000000u
 
-#define BFJ_CHUNK 1024 // candidate records per chunk
+// Candidate records per chunk: one atomic on the chunk counter per 4096 records keeps
+// the join's grabs of fresh chunks off the critical path (the public header's
+// BFJ_CHUNK_RECORDS, asserted equal where both are included).
+#define BFJ_CHUNK 4096
 #ifndef BFJ_PANEL_BYTES
 #define BFJ_PANEL_BYTES (1 << 20) // candidate fragments per join work item
 #endif

@@ -24,7 +24,7 @@
 //   * Z < 0 means "certainly out" for every epsilon.  One max tree and one wave
 //     vote skip almost every tile.  The pairs of a hot tile with Z >= 0 are
 //     appended to a candidate stream (row, col, Z) with wave-compacted plain
-//     stores into per-wavefront chunks of 1024 records (one atomic per chunk),
+//     stores into per-wavefront chunks of 4096 records (one atomic per chunk),
 //     so the hot path never waits on memory or on another wavefront; the
 //     per-epsilon tests, counting and band selection run afterwards in
 //     k_classify (refine.hip).  The reference instead does one global atomic

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 14
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.bfj_abi_version() == bfj.ABI_VERSION == 4
+    assert lib.bfj_abi_version() == bfj.ABI_VERSION == 5
 
 
 def test_cli_exports_reference_entry_points():
@@ -41,6 +41,12 @@ def test_cli_exports_reference_entry_points():
     for sym in ("GPUJoinMainBruteForce(", "GPUJoinMainBruteForceNvidia(", "GPUJoinMainBruteForceEx(",
                 "importDataset(", "importDatasetDim(", "importDatasetBinary(", "exportDatasetBinary("):
         assert sym in out, sym
+
+
+def test_chunk_size_mirror_matches_the_header():
+    txt = open(HEADER).read()
+    m = re.search(r"^#define BFJ_CHUNK_RECORDS (\d+)", txt, re.M)
+    assert m and int(m.group(1)) == bfj.CHUNK_RECORDS
 
 
 def test_stats_mirror_matches_the_header():

@@ -18,16 +18,16 @@ from tensor_bf_main_amd import workloads as wl
 pytestmark = pytest.mark.gpu
 DTYPES = ["fp64", "fp16", "bf16", "fp8"]
 
-CHUNK_BYTES = 1024 * 16  # one chunk of 1024 candidate records
+C = bfj.CHUNK_RECORDS  # candidate records (16 bytes each) per chunk
 
 
 def _limit_for(stats, dtype, pieces):
     """A stream limit whose slices hold about 1/pieces of the pass's candidate chunks:
-    per chunk 16 KiB of records and 4/3 x 1024 + 64 band slots of 16 B per band list,
+    per chunk 16C bytes of records and 4/3 x C + 64 band slots of 16 B per band list,
     plus 1025 slots per list.  At pieces >= 5 the pass in one piece does not fit it."""
     lists = 1 if dtype == "fp64" else 2
-    chunks = max(1, stats["candidates"] // 1024 // pieces)
-    per_chunk = CHUNK_BYTES + lists * 16 * (1024 * 4 / 3 + 64)
+    chunks = max(1, stats["candidates"] // C // pieces)
+    per_chunk = 16 * C + lists * 16 * (C * 4 / 3 + 64)
     return int(lists * 16 * 1025 + chunks * per_chunk) + 1
 
 
@@ -142,7 +142,9 @@ def test_limit_too_small_is_refused(own_ctx):
     """A limit below one chunk and its band slots, then one that a single work item's
     candidates exceed: both fail with BFJ_ERR_CAPACITY and leave the context usable."""
     x, eps = wl.config(3, n=4000)
-    for limit, what in ((40_000, "stream limit"), (2 * 16 * 1025 + 2 * 63_000, "one join work item")):
+    per_chunk = 16 * C + 2 * 16 * (C * 4 / 3 + 64)
+    for limit, what in ((2 * 16 * 1025 + int(0.5 * per_chunk), "stream limit"),
+                        (2 * 16 * 1025 + int(2.03 * per_chunk), "one join work item")):
         own_ctx.set_stream_limit(limit)
         with pytest.raises(bfj.JoinError) as e:
             own_ctx.self_join(x, eps, dtype="fp16")
