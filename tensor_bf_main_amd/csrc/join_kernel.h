@@ -323,21 +323,28 @@ __device__ __forceinline__ void join_strip(const JoinArgs &a, const SetDesc &s, 
             // Padding columns need no mask: their n_j is 1e30 (prep), beyond any
             // acc = g_i + xhat_i . 0.
             const uint32_t colrec = (uint32_t)(s.row_off + cb * 16 + (lane & 15)) | (sym ? BFJ_COL_SYM : 0u);
-            while (hot) {
-                const int rb = __builtin_ctz(hot);
-                hot &= hot - 1u;
-                const int blk = wrb0 + rb;
-                acc_t v = acc[0];
+            // The diagonal block keeps only col > row: one lane mask per register (fixed
+            // per lane), applied under a uniform branch.
+            auto push = [&](acc_t v, int blk) {
+                if (__builtin_expect(sym && blk == cb, 0)) {
 #pragma unroll
-                for (int k = 1; k < RB; ++k)
-                    if (rb == k) v = acc[k];
-                // The diagonal block keeps only col > row: one lane mask per register
-                // (fixed per lane) and'ed with a uniform flag.
-                const bool diag = sym && blk == cb;
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    v[r] = (diag && (lane & 15) <= Tr::out_row(lane, r)) ? -(val_t)__builtin_huge_val() : v[r];
+                    for (int r = 0; r < 4; ++r)
+                        v[r] = ((lane & 15) <= Tr::out_row(lane, r)) ? -(val_t)__builtin_huge_val() : v[r];
+                }
                 cand_push_block<Tr>(a, cc, v, nj, (uint32_t)(s.row_off + blk * 16), colrec, lane);
+            };
+            if (RB == 1) {
+                if (hot) push(acc[0], wrb0);
+            } else {
+                while (hot) {
+                    const int rb = __builtin_ctz(hot);
+                    hot &= hot - 1u;
+                    acc_t v = acc[0];
+#pragma unroll
+                    for (int k = 1; k < RB; ++k)
+                        if (rb == k) v = acc[k];
+                    push(v, wrb0 + rb);
+                }
             }
         }
     };
