@@ -121,9 +121,9 @@ __device__ inline void count_rows(const PassArgs &a, CountTable &tab, bool use_t
                                   int bi, uint32_t j, int bj, int lane)
 {
     if (use_tab) {
-        // Room for this call's keys (at most 2 per lane) at under 5/8 load.
+        // Room for this call's keys (at most 2 per lane) at under 3/4 load.
         if (__hip_atomic_load(&tab.n_used, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) >
-            (uint32_t)(BFJ_CTAB * 5 / 8 - 2 * BFJ_WAVE))
+            (uint32_t)(BFJ_CTAB * 3 / 4 - 2 * BFJ_WAVE))
             ctab_flush(a, tab, lane);
         ctab_add(a, tab, (uint32_t)(s.cnt_off + (int64_t)bi * s.npad + i), bi >= 0);
         ctab_add(a, tab, (uint32_t)(s.cnt_off + (int64_t)bj * s.npad + j), bj >= 0);
